@@ -1,0 +1,131 @@
+/*
+ * sjcs.h -- C-ABI of the MI355X spatial-join-count-over-shells engine.
+ *
+ * The reference (astr_rcount of ryumt/SpatialJoinCountOverShells) has no FFI:
+ * its hot path sits behind the ArraySTR index API and the per-halo
+ * multi-range count.  This header replaces those, one entry point each,
+ * with plain pointers and sizes:
+ *
+ *   reference                                         here
+ *   constructArraySTR        ArraySTR.c:16-34          sjcs_index_create
+ *   insertDataArraySTR  x n  Build.c:702-714,          sjcs_index_insert
+ *                            astr_rcount/main.c:774-778
+ *   makeTree*ArraySTR        Build.c:716-963           sjcs_index_build
+ *   threadJoin + multiRangeCountWithPriodicBoundArraySTR
+ *                            astr_rcount/main.c:795-826,
+ *                            ArraySTR/MultiRangeCount.c:13-73
+ *                                                      sjcs_index_count
+ *   destructArraySTR         ArraySTR.c:36-41          sjcs_index_destroy
+ *
+ * and adds the chunk engine the drop-in CLI drives: the halo set and shell
+ * thresholds stay on the device, each particle file is indexed and counted
+ * on arrival, and the u64 counts accumulate on the device across files
+ * (the reference's per-file loop, astr_rcount/main.c:235-264, with its
+ * counts calloc'ed once at main.c:651 and never reset between files).
+ *
+ * Conventions.
+ *  - Every call returns SJCS_OK (0) or a negative SJCS_ERR_*; nothing
+ *    exits the process.  sjcs_last_error() gives the message of the last
+ *    failure on that engine (or of the last failed sjcs_engine_create when
+ *    passed NULL).
+ *  - Host buffers belong to the caller and are copied before the call
+ *    returns; device buffers passed to the *_device variants must stay
+ *    valid until sjcs_sync() returns.
+ *  - Shell thresholds are the reference's mrcc->radiuses after squaring
+ *    (astr_rcount/main.c:646-649): FP32, one per shell, normally ascending.
+ *    A pair lands in the first shell k with dist < radiuses_sq[k]
+ *    (ArraySTR/MultiRangeCount.c:107-113).
+ *  - Counts are uint64, row-major [nhalos][nrads] (main.c:651,821).
+ *  - One engine per host thread at a time; engines are independent.
+ */
+#ifndef SJCS_H
+#define SJCS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SJCS_OK 0
+#define SJCS_ERR_INVALID (-1) /* bad argument */
+#define SJCS_ERR_DEVICE (-2)  /* HIP runtime or kernel failure */
+#define SJCS_ERR_NOMEM (-3)   /* device or pinned host allocation failed */
+#define SJCS_ERR_STATE (-4)   /* call out of order (e.g. no shells set) */
+
+typedef struct sjcs_engine sjcs_engine;
+typedef struct sjcs_index sjcs_index;
+
+/* ---- engine: device context, shells, halos, device-side accumulation ---- */
+
+/* Opens an engine on HIP device `device`.  No reference counterpart: the
+ * reference's state lives in main()'s JoinContext (astr_rcount/main.c:48-60). */
+int sjcs_engine_create(int device, sjcs_engine **out);
+void sjcs_engine_destroy(sjcs_engine *e);
+const char *sjcs_last_error(const sjcs_engine *e);
+
+/* Shell thresholds (squared radii).  Replaces the threshold template
+ * mrcc->radiuses (astr_rcount/main.c:645-649).  Resets the counts. */
+int sjcs_set_shells(sjcs_engine *e, const float *radiuses_sq, int nrads);
+
+/* Halo centres: nhalos points of 3 FP32 each, `stride_bytes` apart (24 for
+ * the reference's Halo{u64 id; f32 mass; f32 point[3]} with point at byte
+ * 12, include/Gadget/BufHalo.h:26-30; 12 for packed xyz).  Replaces the
+ * BufHalo handed to threadJoin (main.c:635-642).  Resets the counts. */
+int sjcs_set_halos(sjcs_engine *e, const float *points, size_t stride_bytes,
+                   uint64_t nhalos);
+int sjcs_set_halos_device(sjcs_engine *e, const float *d_xyz, uint64_t nhalos);
+
+/* Zeroes the device-side counts. */
+int sjcs_reset_counts(sjcs_engine *e);
+
+/* One particle chunk (packed FP32 xyz, the decoded particle file format,
+ * src/Gadget/BufVector3D.c:285-308): builds the cell index and adds every
+ * halo's shell counts for these particles to the device-side totals.
+ * Replaces one iteration of noBacklogExecution (main.c:246-262):
+ * constructIndexWithSameThreadPool + threadJoin fan-out + destructIndex.
+ * Asynchronous: returns after the upload has been queued. */
+int sjcs_add_particles(sjcs_engine *e, const float *xyz, uint64_t n);
+int sjcs_add_particles_device(sjcs_engine *e, const float *d_xyz, uint64_t n);
+
+/* Waits for all queued work of this engine. */
+int sjcs_sync(sjcs_engine *e);
+
+/* Copies the accumulated counts [nhalos][nrads] to `counts` (host).
+ * Synchronises first. */
+int sjcs_get_counts(sjcs_engine *e, uint64_t *counts);
+
+/* Exact number of in-box (halo image, particle) pair tests the last chunk
+ * made -- the reference's CalcDistance calls (MultiRangeCount.c:107).  Runs
+ * one extra untimed pass; a measurement aid, not part of the count. */
+int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
+
+/* Tuning knobs (no reference counterpart).  Keys:
+ *   "cell_size_milli"   cell edge in 1/1000 box units, 0 = automatic
+ *   "kernel"            0 = automatic, 1 = one halo per wavefront (plain),
+ *                       2 = batched work-queue kernel
+ * Returns SJCS_ERR_INVALID for an unknown key or value. */
+int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
+
+/* Milliseconds the device spent on the last chunk, by phase (HIP events):
+ * [0] upload, [1] index build (key + sort + cell table), [2] count. */
+int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);
+
+/* ---- index: the ArraySTR lifecycle, one device-resident particle index ---- */
+
+int sjcs_index_create(sjcs_engine *e, uint64_t ndata, sjcs_index **out);
+int sjcs_index_insert(sjcs_index *ix, const float *xyz, uint64_t n);
+int sjcs_index_build(sjcs_index *ix);
+/* counts[h*nrads + k] += shell-k count of centre h, for n centres given as
+ * in sjcs_set_halos.  Uses its own shells; leaves the engine's untouched. */
+int sjcs_index_count(sjcs_index *ix, const float *centrals, size_t stride_bytes,
+                     uint64_t n, const float *radiuses_sq, int nrads,
+                     uint64_t *counts);
+void sjcs_index_destroy(sjcs_index *ix);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SJCS_H */

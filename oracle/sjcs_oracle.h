@@ -1,0 +1,24 @@
+/*
+ * sjcs_oracle.h -- CPU restatement of astr_rcount's shell count.
+ * TEST INFRASTRUCTURE ONLY: the checker, never linked by the product.
+ */
+#ifndef SJCS_ORACLE_H
+#define SJCS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+void oracle_shell_thresholds(double minrad, double maxrad, int nrads,
+                             float *radiuses, float *r2);
+void oracle_shell_thresholds_nofma(double minrad, double maxrad, int nrads,
+                                   float *radiuses, float *r2);
+float oracle_box_radius(const float *r2, int nrads);
+float oracle_distance(const float p[3], const float c[3]);
+int oracle_shell_of(float dist, const float *r2, int nrads);
+int oracle_images(const float c[3], float rbox, float centres[8][3]);
+uint64_t oracle_count(const float *particles, uint64_t np,
+                      const void *halo_points, size_t stride_bytes, uint64_t nh,
+                      const float *r2, int nrads, uint64_t *counts,
+                      int nthreads);
+
+#endif

@@ -1,0 +1,268 @@
+"""MI355X spatial-join-count-over-shells engine: Python host mirror.
+
+Thin ctypes bindings over the C-ABI in ``include/sjcs.h`` (library
+``lib/libsjcs.so``, built in-tree by ``make -C spatialjoincountovershells_amd``).
+The names follow the reference's interface for this path
+(src/astr_rcount/main.c, include/objects/ArraySTR/ArraySTR.h):
+
+* :class:`Engine` -- the JoinContext of ``astr_rcount``: shell thresholds
+  (``mrcc->radiuses`` squared, main.c:645-649), the halo set (main.c:635-642)
+  and u64 counts that accumulate over particle chunks (main.c:246-262, 651).
+* :class:`ArraySTR` -- the index lifecycle ``constructArraySTR`` /
+  ``insertDataArraySTR`` / ``makeTree*ArraySTR`` /
+  ``multiRangeCountWithPriodicBoundArraySTR`` / ``destructArraySTR``.
+* :func:`shell_thresholds` -- the CLI's ``--radius min:max:K`` to squared
+  FP32 thresholds (include/support_functions.h:18-34, main.c:506-533).
+
+There is no CPU fallback: without the built library or without a HIP
+device every computing call raises :class:`SJCSError`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsjcs.so")
+HOST_LIB_PATH = os.path.join(_HERE, "lib", "libsjcs_host.so")
+CLI_PATH = os.path.join(_HERE, "bin", "sjcs_rcount")
+
+SJCS_OK = 0
+SJCS_ERR_INVALID = -1
+SJCS_ERR_DEVICE = -2
+SJCS_ERR_NOMEM = -3
+SJCS_ERR_STATE = -4
+
+# Every symbol include/sjcs.h declares.
+EXPORTED = (
+    "sjcs_engine_create", "sjcs_engine_destroy", "sjcs_last_error",
+    "sjcs_set_shells", "sjcs_set_halos", "sjcs_set_halos_device",
+    "sjcs_reset_counts", "sjcs_add_particles", "sjcs_add_particles_device",
+    "sjcs_sync", "sjcs_get_counts", "sjcs_last_chunk_pair_tests",
+    "sjcs_set_option", "sjcs_last_chunk_timings",
+    "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
+    "sjcs_index_count", "sjcs_index_destroy",
+)
+
+
+class SJCSError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"sjcs error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads lib/libsjcs.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise SJCSError(SJCS_ERR_STATE,
+                        f"{path} is not built; run `make -C {_HERE}` first")
+    lib = ctypes.CDLL(path)
+    vp, u64, i32, sz = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "sjcs_engine_create": (i32, [i32, pp]),
+        "sjcs_engine_destroy": (None, [vp]),
+        "sjcs_last_error": (ctypes.c_char_p, [vp]),
+        "sjcs_set_shells": (i32, [vp, vp, i32]),
+        "sjcs_set_halos": (i32, [vp, vp, sz, u64]),
+        "sjcs_set_halos_device": (i32, [vp, vp, u64]),
+        "sjcs_reset_counts": (i32, [vp]),
+        "sjcs_add_particles": (i32, [vp, vp, u64]),
+        "sjcs_add_particles_device": (i32, [vp, vp, u64]),
+        "sjcs_sync": (i32, [vp]),
+        "sjcs_get_counts": (i32, [vp, vp]),
+        "sjcs_last_chunk_pair_tests": (i32, [vp, ctypes.POINTER(u64)]),
+        "sjcs_set_option": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
+        "sjcs_last_chunk_timings": (i32, [vp, ctypes.POINTER(ctypes.c_float)]),
+        "sjcs_index_create": (i32, [vp, u64, pp]),
+        "sjcs_index_insert": (i32, [vp, vp, u64]),
+        "sjcs_index_build": (i32, [vp]),
+        "sjcs_index_count": (i32, [vp, vp, sz, u64, vp, i32, vp]),
+        "sjcs_index_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _points(a, what: str) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError(f"{what} must have shape (n, 3)")
+    return a
+
+
+def _shells(r2) -> np.ndarray:
+    r2 = np.ascontiguousarray(r2, dtype=np.float32).reshape(-1)
+    if r2.size == 0:
+        raise ValueError("at least one shell")
+    return r2
+
+
+class Engine:
+    """Device-resident halos, shells and u64 counts; one particle chunk at a
+    time (the reference's per-file loop, src/astr_rcount/main.c:246-262)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.sjcs_engine_create(device, ctypes.byref(h))
+        if rc != SJCS_OK:
+            raise SJCSError(rc, self._lib.sjcs_last_error(None).decode())
+        self._h = h
+        self.nrads = 0
+        self.nhalos = 0
+
+    def _check(self, rc):
+        if rc != SJCS_OK:
+            raise SJCSError(rc, self._lib.sjcs_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sjcs_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_shells(self, radiuses_sq):
+        r2 = _shells(radiuses_sq)
+        self._check(self._lib.sjcs_set_shells(self._h, _ptr(r2), r2.size))
+        self.nrads = r2.size
+
+    def set_halos(self, points):
+        p = _points(points, "halo points")
+        self._check(self._lib.sjcs_set_halos(self._h, _ptr(p), 12, p.shape[0]))
+        self.nhalos = p.shape[0]
+
+    def set_halos_device(self, ptr: int, n: int):
+        self._check(self._lib.sjcs_set_halos_device(self._h, ctypes.c_void_p(ptr), n))
+        self.nhalos = n
+
+    def reset_counts(self):
+        self._check(self._lib.sjcs_reset_counts(self._h))
+
+    def add_particles(self, xyz):
+        p = _points(xyz, "particles")
+        self._check(self._lib.sjcs_add_particles(self._h, _ptr(p), p.shape[0]))
+
+    def add_particles_device(self, ptr: int, n: int):
+        self._check(self._lib.sjcs_add_particles_device(self._h, ctypes.c_void_p(ptr), n))
+
+    def sync(self):
+        self._check(self._lib.sjcs_sync(self._h))
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros((self.nhalos, self.nrads), dtype=np.uint64)
+        self._check(self._lib.sjcs_get_counts(self._h, _ptr(out)))
+        return out
+
+    def last_chunk_pair_tests(self) -> int:
+        v = ctypes.c_uint64()
+        self._check(self._lib.sjcs_last_chunk_pair_tests(self._h, ctypes.byref(v)))
+        return v.value
+
+    def last_chunk_timings(self):
+        ms = (ctypes.c_float * 3)()
+        self._check(self._lib.sjcs_last_chunk_timings(self._h, ms))
+        return tuple(ms)
+
+    def set_option(self, key: str, value: int):
+        self._check(self._lib.sjcs_set_option(self._h, key.encode(), int(value)))
+
+
+class ArraySTR:
+    """The reference index lifecycle on the device (include/objects/ArraySTR/
+    ArraySTR.h:52-58): construct(n) -> insert(points) -> make_tree() ->
+    multi_range_count(centrals, radiuses_sq) -> close()."""
+
+    def __init__(self, engine: Engine, ndata: int):
+        self._e = engine
+        self._lib = engine._lib
+        h = ctypes.c_void_p()
+        engine._check(self._lib.sjcs_index_create(engine._h, ndata, ctypes.byref(h)))
+        self._h = h
+
+    def insert(self, xyz):
+        p = _points(xyz, "particles")
+        self._e._check(self._lib.sjcs_index_insert(self._h, _ptr(p), p.shape[0]))
+
+    def make_tree(self):
+        self._e._check(self._lib.sjcs_index_build(self._h))
+
+    def multi_range_count(self, centrals, radiuses_sq, counts=None) -> np.ndarray:
+        c = _points(centrals, "centrals")
+        r2 = _shells(radiuses_sq)
+        if counts is None:
+            counts = np.zeros((c.shape[0], r2.size), dtype=np.uint64)
+        if counts.dtype != np.uint64 or not counts.flags.c_contiguous or \
+                counts.shape != (c.shape[0], r2.size):
+            raise ValueError("counts must be C-contiguous uint64 (n, nrads)")
+        self._e._check(self._lib.sjcs_index_count(self._h, _ptr(c), 12, c.shape[0], _ptr(r2),
+                                                  r2.size, _ptr(counts)))
+        return counts
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sjcs_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def shell_thresholds(minrad: float, maxrad: float, nrads: int):
+    """(radiuses, radiuses_sq) as the reference computes them for
+    ``--radius minrad:maxrad:nrads`` after its min/max swap
+    (include/support_functions.h:18-34 with the exponent a + i*c evaluated
+    as fma(i, c, a), as gcc -O2 compiles it on an FMA host; squares in FP32,
+    src/astr_rcount/main.c:648-649).  Computed by the C host library."""
+    host = load_host_library()
+    r = np.zeros(nrads, dtype=np.float32)
+    r2 = np.zeros(nrads, dtype=np.float32)
+    host.sjcs_host_shell_thresholds(ctypes.c_double(minrad), ctypes.c_double(maxrad), nrads,
+                                    _ptr(r), _ptr(r2))
+    return r, r2
+
+
+_host = None
+
+
+def load_host_library(path: str = HOST_LIB_PATH) -> ctypes.CDLL:
+    """The C host library (CLI parsing, file formats, thresholds, output)."""
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(path):
+        raise SJCSError(SJCS_ERR_STATE,
+                        f"{path} is not built; run `make -C {_HERE}` first")
+    lib = ctypes.CDLL(path)
+    lib.sjcs_host_shell_thresholds.restype = None
+    lib.sjcs_host_shell_thresholds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_void_p, ctypes.c_void_p]
+    _host = lib
+    return lib

@@ -1,0 +1,700 @@
+// engine.hip -- the C-ABI (include/sjcs.h): device state, chunk pipeline,
+// and the ArraySTR-shaped index API.
+//
+// Per particle chunk (one reference particle file, src/astr_rcount/main.c:
+// 246-262): upload -> cell keys -> LSD radix sort -> cell table -> count of
+// every halo, added to u64 counts that stay on the device until the caller
+// asks for them.  There is no CPU fallback: every entry point that computes
+// needs a HIP device and fails with SJCS_ERR_DEVICE without one.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sjcs.h"
+#include "common.h"
+#include "count.h"
+#include "primitives.h"
+
+using namespace sjcs;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Grow-only device buffer.
+hipError_t ensure(DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return hipSuccess;
+  b.release();
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return e;
+  }
+  b.bytes = bytes;
+  return hipSuccess;
+}
+
+struct ShellState {
+  int nrads = 0;
+  std::vector<float> r2;
+  DevBuf d_r2, d_lut;
+  Shells view{};
+};
+
+struct HaloState {
+  uint64_t n = 0;
+  DevBuf x, y, z, row;
+};
+
+// A built particle index: sorted SoA coordinates and the cell table, plus
+// the sort's ping-pong buffers.
+struct IndexBuffers {
+  DevBuf key[2], x[2], y[2], z[2];
+  DevBuf cell_end;
+  Grid grid{};
+  uint64_t n = 0;
+  int sel = 0;
+  void release() {
+    for (int i = 0; i < 2; ++i) {
+      key[i].release();
+      x[i].release();
+      y[i].release();
+      z[i].release();
+    }
+    cell_end.release();
+  }
+  IndexView view() const {
+    IndexView v;
+    v.x = x[sel].as<float>();
+    v.y = y[sel].as<float>();
+    v.z = z[sel].as<float>();
+    v.cell_end = cell_end.as<uint32_t>();
+    v.grid = grid;
+    v.n = n;
+    return v;
+  }
+};
+
+int key_bits_for(uint32_t ncells) {
+  int b = 0;
+  while (b < 32 && (1ull << b) < (uint64_t)ncells) ++b;
+  return b;
+}
+
+// Cell edge: half the query half-width by default (a query box then spans
+// about 5 cells per axis), coarsened until there are at most ~2 cells per
+// particle, and at most 1024 cells per axis.
+Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli) {
+  double s;
+  if (cell_milli > 0)
+    s = (double)cell_milli / 1000.0;
+  else if (rbox > 0.0f && isfinite(rbox))
+    s = (double)rbox / 2.0;
+  else
+    s = 1000.0;
+  double gd = ceil(1000.0 / s);
+  if (!(gd >= 1.0)) gd = 1.0;
+  if (gd > 1024.0) gd = 1024.0;
+  double cap = cbrt((double)std::max<uint64_t>(4096, 2 * n));
+  if (cell_milli <= 0 && gd > cap) gd = floor(cap);
+  if (gd < 1.0) gd = 1.0;
+  Grid g;
+  g.g = (int)gd;
+  g.inv_cell = (float)g.g / 1000.0f;
+  g.ncells = (uint32_t)g.g * (uint32_t)g.g * (uint32_t)g.g;
+  return g;
+}
+
+uint32_t float_bits(float f) {
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  return b;
+}
+
+float bits_float(uint32_t b) {
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+}  // namespace
+
+struct sjcs_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  bool have_shells = false;
+  ShellState shells;
+  bool have_halos = false;
+  HaloState halos;
+
+  DevBuf counts;
+  bool counts_fresh = true;  // no chunk since the last reset: next one stores
+
+  IndexBuffers ib;
+  DevBuf in;  // AoS chunk staging on the device
+  DevBuf ws;  // sort / scan workspace
+  DevBuf inbox;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+
+  int64_t cell_size_milli = 0;
+  int64_t kernel = 0;
+
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed = false;
+};
+
+struct sjcs_index {
+  sjcs_engine* e = nullptr;
+  uint64_t cap = 0;
+  uint64_t n = 0;
+  bool built = false;
+  DevBuf aos;
+  IndexBuffers ib;
+};
+
+namespace {
+
+int fail(sjcs_engine* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+int hip_fail(sjcs_engine* e, hipError_t h, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(h);
+  return fail(e, h == hipErrorOutOfMemory ? SJCS_ERR_NOMEM : SJCS_ERR_DEVICE, m);
+}
+
+#define HIPCALL(e, expr, what)                          \
+  do {                                                  \
+    hipError_t h_ = (expr);                             \
+    if (h_ != hipSuccess) return hip_fail((e), h_, (what)); \
+  } while (0)
+
+int use_device(sjcs_engine* e) {
+  HIPCALL(e, hipSetDevice(e->device), "hipSetDevice");
+  return SJCS_OK;
+}
+
+// Pinned host staging of at least `bytes`; waits for the stream first so a
+// previous upload from it has finished.
+int ensure_pinned(sjcs_engine* e, size_t bytes) {
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  if (e->pinned_bytes >= bytes) return SJCS_OK;
+  if (e->pinned) (void)hipHostFree(e->pinned);
+  e->pinned = nullptr;
+  e->pinned_bytes = 0;
+  HIPCALL(e, hipHostMalloc(&e->pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
+  e->pinned_bytes = bytes;
+  return SJCS_OK;
+}
+
+int ensure_ws(sjcs_engine* e, uint64_t words) {
+  HIPCALL(e, ensure(e->ws, words * 4), "hipMalloc(workspace)");
+  return SJCS_OK;
+}
+
+// Validates and uploads shell thresholds; builds the first-match table.
+int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
+  if (!r2 || nrads <= 0) return fail(e, SJCS_ERR_INVALID, "nrads must be >= 1");
+  st.nrads = nrads;
+  st.r2.assign(r2, r2 + nrads);
+  bool sorted = nrads <= 65535;
+  for (int k = 0; k < nrads && sorted; ++k) {
+    uint32_t b = float_bits(r2[k]);
+    if (b >= 0x7f800001u) sorted = false;  // NaN or sign bit set
+    if (k > 0 && !(r2[k - 1] <= r2[k])) sorted = false;
+  }
+  Shells v{};
+  v.nrads = nrads;
+  v.sorted = sorted ? 1 : 0;
+  v.r2max = r2[nrads - 1];
+  // src/objects/ArraySTR/MultiRangeCount.c:19: float radius = sqrt(r2[K-1])
+  v.rbox = (float)sqrt((double)r2[nrads - 1]);
+  std::vector<uint16_t> lut(1, 0);
+  if (sorted) {
+    const uint32_t base = float_bits(r2[0]);
+    const uint32_t range = float_bits(r2[nrads - 1]) - base;
+    int shift = 0;
+    while ((range >> shift) >= (uint32_t)kLutSize) ++shift;
+    const int nlut = (int)(range >> shift) + 1;
+    lut.assign(nlut, 0);
+    for (int b = 0; b < nlut; ++b) {
+      // smallest distance that maps to bucket b (bucket 0 also takes all
+      // distances below r2[0], down to +0)
+      const float dmin = b == 0 ? 0.0f : bits_float(base + ((uint32_t)b << shift));
+      int k = 0;
+      while (k < nrads - 1 && !(dmin < r2[k])) ++k;
+      lut[b] = (uint16_t)k;
+    }
+    v.lut_base = base;
+    v.lut_shift = shift;
+    v.nlut = nlut;
+  }
+  HIPCALL(e, ensure(st.d_r2, sizeof(float) * nrads), "hipMalloc(shells)");
+  HIPCALL(e, ensure(st.d_lut, sizeof(uint16_t) * lut.size()), "hipMalloc(shell table)");
+  HIPCALL(e, hipMemcpyAsync(st.d_r2.p, r2, sizeof(float) * nrads, hipMemcpyHostToDevice, e->stream),
+          "hipMemcpy(shells)");
+  HIPCALL(e, hipMemcpyAsync(st.d_lut.p, lut.data(), sizeof(uint16_t) * lut.size(),
+                            hipMemcpyHostToDevice, e->stream),
+          "hipMemcpy(shell table)");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  v.r2 = st.d_r2.as<float>();
+  v.lut = st.d_lut.as<uint16_t>();
+  st.view = v;
+  return SJCS_OK;
+}
+
+// Halo centres from a device AoS (packed xyz) -> Morton-sorted SoA + rows.
+int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n) {
+  if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 halos");
+  hs.n = n;
+  const size_t b = sizeof(uint32_t) * (n ? n : 1);
+  DevBuf key[2], x[2], y[2], z[2], row[2];
+  int rc = SJCS_OK;
+  hipError_t h = hipSuccess;
+  for (int i = 0; i < 2 && h == hipSuccess; ++i) {
+    h = ensure(key[i], b);
+    if (h == hipSuccess) h = ensure(x[i], b);
+    if (h == hipSuccess) h = ensure(y[i], b);
+    if (h == hipSuccess) h = ensure(z[i], b);
+    if (h == hipSuccess) h = ensure(row[i], b);
+  }
+  if (h == hipSuccess) rc = ensure_ws(e, radix_workspace_words(n));
+  if (h == hipSuccess && rc == SJCS_OK)
+    h = halo_keys(d_xyz, n, key[0].as<uint32_t>(), x[0].as<float>(), y[0].as<float>(),
+                  z[0].as<float>(), row[0].as<uint32_t>(), e->stream);
+  int sel = 0;
+  if (h == hipSuccess && rc == SJCS_OK) {
+    uint32_t* keys[2] = {key[0].as<uint32_t>(), key[1].as<uint32_t>()};
+    uint32_t* pay[2][4] = {
+        {x[0].as<uint32_t>(), y[0].as<uint32_t>(), z[0].as<uint32_t>(), row[0].as<uint32_t>()},
+        {x[1].as<uint32_t>(), y[1].as<uint32_t>(), z[1].as<uint32_t>(), row[1].as<uint32_t>()}};
+    h = radix_sort_pairs(keys, pay, 4, n, 30, e->ws.as<uint32_t>(), e->stream, &sel);
+  }
+  if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+  if (h != hipSuccess) rc = hip_fail(e, h, "halo sort");
+  if (rc == SJCS_OK) {
+    hs.x.release();
+    hs.y.release();
+    hs.z.release();
+    hs.row.release();
+    hs.x = x[sel];
+    hs.y = y[sel];
+    hs.z = z[sel];
+    hs.row = row[sel];
+    x[sel] = DevBuf();
+    y[sel] = DevBuf();
+    z[sel] = DevBuf();
+    row[sel] = DevBuf();
+  }
+  for (int i = 0; i < 2; ++i) {
+    key[i].release();
+    x[i].release();
+    y[i].release();
+    z[i].release();
+    row[i].release();
+  }
+  return rc;
+}
+
+// Packs strided host points into pinned memory and uploads them to `dst`.
+int upload_points(sjcs_engine* e, DevBuf& dst, const float* pts, size_t stride, uint64_t n) {
+  const size_t bytes = 12 * (size_t)n;
+  HIPCALL(e, ensure(dst, bytes), "hipMalloc(points)");
+  if (n == 0) return SJCS_OK;
+  int rc = ensure_pinned(e, bytes);
+  if (rc != SJCS_OK) return rc;
+  float* p = static_cast<float*>(e->pinned);
+  if (stride == 12) {
+    memcpy(p, pts, bytes);
+  } else {
+    const char* src = reinterpret_cast<const char*>(pts);
+    for (uint64_t i = 0; i < n; ++i) memcpy(p + 3 * i, src + i * stride, 12);
+  }
+  HIPCALL(e, hipMemcpyAsync(dst.p, p, bytes, hipMemcpyHostToDevice, e->stream), "hipMemcpy(points)");
+  return SJCS_OK;
+}
+
+// Builds the cell index of n particles at d_xyz (device, packed) into ib.
+int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g) {
+  if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles per chunk");
+  const size_t b = sizeof(uint32_t) * (n ? n : 1);
+  for (int i = 0; i < 2; ++i) {
+    HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
+    HIPCALL(e, ensure(ib.x[i], b), "hipMalloc(index)");
+    HIPCALL(e, ensure(ib.y[i], b), "hipMalloc(index)");
+    HIPCALL(e, ensure(ib.z[i], b), "hipMalloc(index)");
+  }
+  HIPCALL(e, ensure(ib.cell_end, sizeof(uint32_t) * (size_t)g.ncells), "hipMalloc(cell table)");
+  int rc = ensure_ws(e, std::max(radix_workspace_words(n), scan_workspace_words(g.ncells)));
+  if (rc != SJCS_OK) return rc;
+  ib.grid = g;
+  ib.n = n;
+  HIPCALL(e, particle_keys(d_xyz, n, g, ib.key[0].as<uint32_t>(), ib.x[0].as<float>(),
+                           ib.y[0].as<float>(), ib.z[0].as<float>(), e->stream),
+          "cell keys");
+  uint32_t* keys[2] = {ib.key[0].as<uint32_t>(), ib.key[1].as<uint32_t>()};
+  uint32_t* pay[2][4] = {
+      {ib.x[0].as<uint32_t>(), ib.y[0].as<uint32_t>(), ib.z[0].as<uint32_t>(), nullptr},
+      {ib.x[1].as<uint32_t>(), ib.y[1].as<uint32_t>(), ib.z[1].as<uint32_t>(), nullptr}};
+  int sel = 0;
+  HIPCALL(e, radix_sort_pairs(keys, pay, 3, n, key_bits_for(g.ncells), e->ws.as<uint32_t>(),
+                              e->stream, &sel),
+          "radix sort");
+  ib.sel = sel;
+  HIPCALL(e, build_cell_end(ib.key[sel].as<uint32_t>(), n, ib.cell_end.as<uint32_t>(), g.ncells,
+                            e->ws.as<uint32_t>(), e->stream),
+          "cell table");
+  return SJCS_OK;
+}
+
+HaloView halo_view(const HaloState& h) {
+  HaloView v;
+  v.x = h.x.as<float>();
+  v.y = h.y.as<float>();
+  v.z = h.z.as<float>();
+  v.orig = h.row.as<uint32_t>();
+  v.n = h.n;
+  return v;
+}
+
+int prepare_counts(sjcs_engine* e) {
+  e->counts_fresh = true;
+  if (!e->have_shells || !e->have_halos) return SJCS_OK;
+  const size_t bytes = sizeof(uint64_t) * (size_t)e->halos.n * (size_t)e->shells.nrads;
+  HIPCALL(e, ensure(e->counts, bytes), "hipMalloc(counts)");
+  return SJCS_OK;
+}
+
+int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+  const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli);
+  int rc = build_index(e, e->ib, d_xyz, n, g);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
+  HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view,
+                         e->counts.as<uint64_t>(), !e->counts_fresh, nullptr, e->stream),
+          "count kernel");
+  HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
+  e->counts_fresh = false;
+  e->timed = true;
+  return SJCS_OK;
+}
+
+int check_ready(sjcs_engine* e) {
+  if (!e) return SJCS_ERR_INVALID;
+  if (!e->have_shells) return fail(e, SJCS_ERR_STATE, "no shells set (sjcs_set_shells)");
+  if (!e->have_halos) return fail(e, SJCS_ERR_STATE, "no halos set (sjcs_set_halos)");
+  return use_device(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sjcs_engine_create(int device, sjcs_engine** out) {
+  if (!out) return SJCS_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t h = hipGetDeviceCount(&ndev);
+  if (h != hipSuccess || ndev == 0) {
+    g_create_error = std::string("no HIP device: ") + hipGetErrorString(h);
+    return SJCS_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return SJCS_ERR_INVALID;
+  }
+  sjcs_engine* e = new sjcs_engine();
+  e->device = device;
+  h = hipSetDevice(device);
+  if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  for (int i = 0; i < 4 && h == hipSuccess; ++i) h = hipEventCreate(&e->ev[i]);
+  if (h == hipSuccess) h = ensure(e->inbox, sizeof(unsigned long long));
+  if (h != hipSuccess) {
+    g_create_error = std::string("engine setup: ") + hipGetErrorString(h);
+    sjcs_engine_destroy(e);
+    return SJCS_ERR_DEVICE;
+  }
+  *out = e;
+  return SJCS_OK;
+}
+
+void sjcs_engine_destroy(sjcs_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  e->shells.d_r2.release();
+  e->shells.d_lut.release();
+  e->halos.x.release();
+  e->halos.y.release();
+  e->halos.z.release();
+  e->halos.row.release();
+  e->counts.release();
+  e->ib.release();
+  e->in.release();
+  e->ws.release();
+  e->inbox.release();
+  if (e->pinned) (void)hipHostFree(e->pinned);
+  for (int i = 0; i < 4; ++i)
+    if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* sjcs_last_error(const sjcs_engine* e) {
+  return e ? e->err.c_str() : g_create_error.c_str();
+}
+
+int sjcs_set_shells(sjcs_engine* e, const float* radiuses_sq, int nrads) {
+  if (!e) return SJCS_ERR_INVALID;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  rc = make_shells(e, e->shells, radiuses_sq, nrads);
+  if (rc != SJCS_OK) return rc;
+  e->have_shells = true;
+  return prepare_counts(e);
+}
+
+int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uint64_t nhalos) {
+  if (!e) return SJCS_ERR_INVALID;
+  if ((!points && nhalos) || stride_bytes < 12)
+    return fail(e, SJCS_ERR_INVALID, "halo points: null pointer or stride < 12");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  rc = upload_points(e, e->in, points, stride_bytes, nhalos);
+  if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos);
+  if (rc != SJCS_OK) return rc;
+  e->have_halos = true;
+  return prepare_counts(e);
+}
+
+int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
+  if (!e) return SJCS_ERR_INVALID;
+  if (!d_xyz && nhalos) return fail(e, SJCS_ERR_INVALID, "halo points: null pointer");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  rc = sort_halos(e, e->halos, d_xyz, nhalos);
+  if (rc != SJCS_OK) return rc;
+  e->have_halos = true;
+  return prepare_counts(e);
+}
+
+int sjcs_reset_counts(sjcs_engine* e) {
+  if (!e) return SJCS_ERR_INVALID;
+  return prepare_counts(e);
+}
+
+int sjcs_add_particles(sjcs_engine* e, const float* xyz, uint64_t n) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if (!xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
+  HIPCALL(e, hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
+  rc = upload_points(e, e->in, xyz, 12, n);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
+  return add_chunk(e, e->in.as<float>(), n);
+}
+
+int sjcs_add_particles_device(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if (!d_xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
+  HIPCALL(e, hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
+  return add_chunk(e, d_xyz, n);
+}
+
+int sjcs_sync(sjcs_engine* e) {
+  if (!e) return SJCS_ERR_INVALID;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  return SJCS_OK;
+}
+
+int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  const size_t n = (size_t)e->halos.n * (size_t)e->shells.nrads;
+  if (!counts && n) return fail(e, SJCS_ERR_INVALID, "counts: null pointer");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  if (n == 0) return SJCS_OK;
+  if (e->counts_fresh) {
+    memset(counts, 0, n * sizeof(uint64_t));
+    return SJCS_OK;
+  }
+  HIPCALL(e, hipMemcpy(counts, e->counts.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost),
+          "hipMemcpy(counts)");
+  return SJCS_OK;
+}
+
+int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if (!pair_tests) return fail(e, SJCS_ERR_INVALID, "null output");
+  *pair_tests = 0;
+  if (!e->timed) return SJCS_OK;
+  HIPCALL(e, fill_u64(e->inbox.as<uint64_t>(), 0ull, 1, e->stream), "fill");
+  HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view, nullptr, true,
+                         e->inbox.as<unsigned long long>(), e->stream),
+          "pair-test count");
+  HIPCALL(e, hipMemcpyAsync(pair_tests, e->inbox.p, 8, hipMemcpyDeviceToHost, e->stream),
+          "hipMemcpy");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  return SJCS_OK;
+}
+
+int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
+  if (!e || !key) return SJCS_ERR_INVALID;
+  if (strcmp(key, "cell_size_milli") == 0) {
+    if (value < 0 || (value > 0 && value < 977))  // at most 1024 cells per axis
+      return fail(e, SJCS_ERR_INVALID, "cell_size_milli: 0 or >= 977");
+    e->cell_size_milli = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "kernel") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "kernel: 0 or 1");
+    e->kernel = value;
+    return SJCS_OK;
+  }
+  return fail(e, SJCS_ERR_INVALID, std::string("unknown option ") + key);
+}
+
+int sjcs_last_chunk_timings(sjcs_engine* e, float ms[3]) {
+  if (!e || !ms) return SJCS_ERR_INVALID;
+  ms[0] = ms[1] = ms[2] = 0.0f;
+  if (!e->timed) return SJCS_OK;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  HIPCALL(e, hipEventElapsedTime(&ms[0], e->ev[0], e->ev[1]), "hipEventElapsedTime");
+  HIPCALL(e, hipEventElapsedTime(&ms[1], e->ev[1], e->ev[2]), "hipEventElapsedTime");
+  HIPCALL(e, hipEventElapsedTime(&ms[2], e->ev[2], e->ev[3]), "hipEventElapsedTime");
+  return SJCS_OK;
+}
+
+// ---- index API ----
+
+int sjcs_index_create(sjcs_engine* e, uint64_t ndata, sjcs_index** out) {
+  if (!e || !out) return SJCS_ERR_INVALID;
+  *out = nullptr;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (ndata > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles");
+  sjcs_index* ix = new sjcs_index();
+  ix->e = e;
+  ix->cap = ndata;
+  hipError_t h = ensure(ix->aos, 12 * (size_t)(ndata ? ndata : 1));
+  if (h != hipSuccess) {
+    delete ix;
+    return hip_fail(e, h, "hipMalloc(index)");
+  }
+  *out = ix;
+  return SJCS_OK;
+}
+
+int sjcs_index_insert(sjcs_index* ix, const float* xyz, uint64_t n) {
+  if (!ix) return SJCS_ERR_INVALID;
+  sjcs_engine* e = ix->e;
+  if (!xyz && n) return fail(e, SJCS_ERR_INVALID, "null points");
+  if (ix->built) return fail(e, SJCS_ERR_STATE, "index already built");
+  if (n > ix->cap - ix->n) return fail(e, SJCS_ERR_INVALID, "more points than the index holds");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (n == 0) return SJCS_OK;
+  HIPCALL(e, hipMemcpyAsync(ix->aos.as<char>() + 12 * ix->n, xyz, 12 * (size_t)n,
+                            hipMemcpyHostToDevice, e->stream),
+          "hipMemcpy(points)");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  ix->n += n;
+  return SJCS_OK;
+}
+
+int sjcs_index_build(sjcs_index* ix) {
+  if (!ix) return SJCS_ERR_INVALID;
+  sjcs_engine* e = ix->e;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (ix->built) return SJCS_OK;
+  // no query radius is known yet: size cells for about two particles each
+  const Grid g = choose_grid(ix->n, 0.0f, e->cell_size_milli);
+  rc = build_index(e, ix->ib, ix->aos.as<float>(), ix->n, g);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  ix->built = true;
+  return SJCS_OK;
+}
+
+int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes, uint64_t n,
+                     const float* radiuses_sq, int nrads, uint64_t* counts) {
+  if (!ix) return SJCS_ERR_INVALID;
+  sjcs_engine* e = ix->e;
+  if (!ix->built) return fail(e, SJCS_ERR_STATE, "index not built (sjcs_index_build)");
+  if ((!centrals && n) || stride_bytes < 12 || (!counts && n))
+    return fail(e, SJCS_ERR_INVALID, "null pointer or stride < 12");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (n == 0) return SJCS_OK;
+  ShellState sh;
+  HaloState hs;
+  DevBuf pts, dcounts;
+  rc = make_shells(e, sh, radiuses_sq, nrads);
+  if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
+  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), n);
+  const size_t nc = (size_t)n * (size_t)nrads;
+  std::vector<uint64_t> host;
+  if (rc == SJCS_OK) {
+    hipError_t h = ensure(dcounts, nc * sizeof(uint64_t));
+    if (h == hipSuccess)
+      h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint64_t>(), false,
+                      nullptr, e->stream);
+    if (h == hipSuccess) {
+      host.resize(nc);
+      h = hipMemcpyAsync(host.data(), dcounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         e->stream);
+    }
+    if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+    if (h != hipSuccess) rc = hip_fail(e, h, "index count");
+  }
+  if (rc == SJCS_OK)
+    for (size_t i = 0; i < nc; ++i) counts[i] += host[i];
+  sh.d_r2.release();
+  sh.d_lut.release();
+  hs.x.release();
+  hs.y.release();
+  hs.z.release();
+  hs.row.release();
+  pts.release();
+  dcounts.release();
+  return rc;
+}
+
+void sjcs_index_destroy(sjcs_index* ix) {
+  if (!ix) return;
+  (void)hipSetDevice(ix->e->device);
+  (void)hipStreamSynchronize(ix->e->stream);
+  ix->aos.release();
+  ix->ib.release();
+  delete ix;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// primitives.h -- host launchers for the device-wide building blocks:
+// scans, the LSD radix sort and the cell table.  All launches go to the
+// given stream and allocate nothing; callers pass the workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace sjcs {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per block
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Workspace (u32 words) a radix sort of n keys needs: per-tile digit
+// histograms plus the scan partials over them.
+inline uint64_t radix_workspace_words(uint64_t n) {
+  uint64_t tiles = ceil_div(n ? n : 1, kSortTile);
+  uint64_t hist = (uint64_t)kRadix * tiles;
+  return hist + ceil_div(hist, kScanTile) + 64;
+}
+
+// Workspace (u32 words) for a scan of n words.
+inline uint64_t scan_workspace_words(uint64_t n) {
+  return ceil_div(n ? n : 1, kScanTile) + 64;
+}
+
+// out[i] = in[0] + ... + in[i-1] (exclusive), in place allowed.
+hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n,
+                             uint32_t* ws, hipStream_t s);
+// out[i] = max(in[0..i]) (inclusive), in place allowed.
+hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n,
+                             uint32_t* ws, hipStream_t s);
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+
+// Stable LSD radix sort of (key, payload[npay]) by the low `key_bits` bits
+// of key.  Buffers come in pairs [0]/[1]; data starts in [0]; returns in
+// *result which pair holds the sorted data.  npay is 3 or 4.
+hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay,
+                            uint64_t n, int key_bits, uint32_t* ws,
+                            hipStream_t s, int* result);
+
+// Particle chunk (packed xyz) -> cell keys + SoA coordinates.
+hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key,
+                         float* x, float* y, float* z, hipStream_t s);
+
+// Halo centres (packed xyz) -> Morton keys (10 bits per axis) + SoA + row.
+hipError_t halo_keys(const float* xyz, uint64_t n, uint32_t* key, float* x,
+                     float* y, float* z, uint32_t* row, hipStream_t s);
+
+// Cell table from keys sorted ascending: cell_end[c] = #keys <= c.
+hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n,
+                          uint32_t* cell_end, uint32_t ncells, uint32_t* ws,
+                          hipStream_t s);
+
+}  // namespace sjcs

@@ -1,0 +1,455 @@
+// primitives.hip -- device-wide scans, the LSD radix sort that replaces the
+// reference's STR bulk load, and the cell table.
+//
+// The reference builds its index with a parallel LSD radix sort over FP32
+// coordinate keys, one axis after another, then packs runs of 10 into tree
+// nodes (src/objects/ArraySTR/Build.c:845-963).  Here the index is a cell
+// grid: one stable LSD radix sort of (cell key, x, y, z) by 8-bit digits,
+// then a table of where every cell ends.  Each sort pass is three kernels:
+// per-tile digit histograms, one exclusive scan over them, and a scatter
+// that ranks each tile in LDS (wave ballots give the stable rank) and writes
+// every digit's run out contiguously.
+#include "primitives.h"
+
+namespace sjcs {
+namespace {
+
+struct OpSum {
+  __device__ static __forceinline__ uint32_t apply(uint32_t a, uint32_t b) { return a + b; }
+};
+struct OpMax {
+  __device__ static __forceinline__ uint32_t apply(uint32_t a, uint32_t b) { return a > b ? a : b; }
+};
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+template <class Op>
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v = Op::apply(v, t);
+  }
+  return v;
+}
+
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = Op::apply(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Exclusive block scan; both sum and max have identity 0 on u32.
+template <class Op, int NT>
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t inc = wave_incl_scan<Op>(v, lane);
+  uint32_t exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = 0;
+  if (lane == 63) s_tmp[w] = inc;
+  __syncthreads();
+  uint32_t prefix = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    uint32_t t = s_tmp[i];
+    if (i < w) prefix = Op::apply(prefix, t);
+    tot = Op::apply(tot, t);
+  }
+  __syncthreads();
+  *total = tot;
+  return Op::apply(prefix, exc);
+}
+
+// ---------------------------------------------------------------- scans ----
+
+template <class Op>
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
+    const uint32_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_tmp[kScanThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    if (i < n) acc = Op::apply(acc, in[i]);
+  }
+  acc = wave_reduce<Op>(acc);
+  if (lane_id() == 0) s_tmp[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i) r = Op::apply(r, s_tmp[i]);
+    partials[blockIdx.x] = r;
+  }
+}
+
+// Exclusive scan of the per-tile partials, in place, by one block.
+template <class Op>
+__global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t* p, uint32_t np) {
+  __shared__ uint32_t s_tmp[16];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < np; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < np ? p[i] : 0;
+    uint32_t tot;
+    uint32_t exc = block_excl_scan<Op, 1024>(v, s_tmp, &tot);
+    if (i < np) p[i] = Op::apply(carry, exc);
+    carry = Op::apply(carry, tot);
+  }
+}
+
+template <class Op, bool kExclusive>
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(
+    const uint32_t* in, uint32_t* out, uint64_t n, const uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_tmp[kScanThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  if (base + kScanItems <= n) {
+    const uint4* src = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int j = 0; j < kScanItems / 4; ++j) {
+      uint4 q = src[j];
+      v[4 * j] = q.x; v[4 * j + 1] = q.y; v[4 * j + 2] = q.z; v[4 * j + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < n) ? in[base + j] : 0u;
+  }
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) acc = Op::apply(acc, v[j]);
+  uint32_t tot;
+  uint32_t run = Op::apply(partials[blockIdx.x], block_excl_scan<Op, kScanThreads>(acc, s_tmp, &tot));
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    uint32_t x = v[j];
+    if (kExclusive) {
+      v[j] = run;
+      run = Op::apply(run, x);
+    } else {
+      run = Op::apply(run, x);
+      v[j] = run;
+    }
+  }
+  if (base + kScanItems <= n) {
+    uint4* dst = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+    for (int j = 0; j < kScanItems / 4; ++j)
+      dst[j] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+      if (base + j < n) out[base + j] = v[j];
+  }
+}
+
+template <class Op, bool kExclusive>
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* ws, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t tiles = ceil_div(n, kScanTile);
+  if (tiles > 0xffffffffull) return hipErrorInvalidValue;
+  scan_reduce_kernel<Op><<<(uint32_t)tiles, kScanThreads, 0, s>>>(in, n, ws);
+  scan_partials_kernel<Op><<<1, 1024, 0, s>>>(ws, (uint32_t)tiles);
+  scan_apply_kernel<Op, kExclusive><<<(uint32_t)tiles, kScanThreads, 0, s>>>(in, out, n, ws);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------- fills ----
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+uint32_t fill_blocks(uint64_t n) {
+  uint64_t b = ceil_div(n, 256);
+  return (uint32_t)(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+// ------------------------------------------------------------ radix sort ----
+
+// Mask of the lanes of this wave whose `digit` equals this lane's (among
+// lanes with `valid`); a lane that is not valid gets 0.
+__device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    bool bit = (digit >> b) & 1u;
+    uint64_t bb = __ballot(bit);
+    peers &= bit ? bb : ~bb;
+  }
+  return valid ? peers : 0ull;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane_id()) - 1ull;
+}
+
+// Per-tile digit histogram, digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
+    const uint32_t* __restrict__ keys, uint64_t n, int shift,
+    uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t s_h[kRadix];
+  const int tid = threadIdx.x;
+  s_h[tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
+    bool valid = i < n;
+    uint32_t d = valid ? (keys[i] >> shift) & (kRadix - 1) : 0u;
+    uint64_t peers = match_digit(d, valid);
+    // one LDS add per distinct digit of the wave: sorted or clumped input
+    // would otherwise pile every lane onto one counter
+    if (valid && (peers & lanemask_lt()) == 0)
+      atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  hist[(uint64_t)tid * ntiles + blockIdx.x] = s_h[tid];
+}
+
+template <int NPAY>
+__global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ p0i,
+    const uint32_t* __restrict__ p1i, const uint32_t* __restrict__ p2i,
+    const uint32_t* __restrict__ p3i, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ p0o, uint32_t* __restrict__ p1o,
+    uint32_t* __restrict__ p2o, uint32_t* __restrict__ p3o, uint64_t n,
+    int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t s_keys[kSortTile];
+  __shared__ uint32_t s_pay[NPAY][kSortTile];
+  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
+  __shared__ uint32_t s_woff[kSortThreads / 64][kRadix];
+  __shared__ uint32_t s_run[kRadix];
+  __shared__ uint32_t s_gbase[kRadix];
+  __shared__ uint32_t s_lstart[kRadix];
+  __shared__ uint32_t s_tmp[kSortThreads / 64];
+
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  const uint32_t* pin[4] = {p0i, p1i, p2i, p3i};
+  uint32_t* pout[4] = {p0o, p1o, p2o, p3o};
+
+  s_run[tid] = 0;
+  s_gbase[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][tid] = 0;
+
+  uint32_t key[kSortItems], rank[kSortItems];
+  uint32_t pay[NPAY][kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
+    bool valid = i < n;
+    key[j] = valid ? kin[i] : 0xffffffffu;
+#pragma unroll
+    for (int p = 0; p < NPAY; ++p) pay[p][j] = valid ? pin[p][i] : 0u;
+  }
+  __syncthreads();
+
+  // Stable rank of every item among the items of its digit in this tile:
+  // items are visited in input order (round j covers indices j*256 + tid).
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
+    bool valid = i < n;
+    uint32_t d = (key[j] >> shift) & (kRadix - 1);
+    uint64_t peers = match_digit(d, valid);
+    uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+    if (valid && before == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t run = s_run[tid];
+#pragma unroll
+      for (int w = 0; w < kSortThreads / 64; ++w) {
+        uint32_t c = s_wcnt[w][tid];
+        s_woff[w][tid] = run;
+        run += c;
+        s_wcnt[w][tid] = 0;
+      }
+      s_run[tid] = run;
+    }
+    __syncthreads();
+    rank[j] = valid ? s_woff[wave][d] + before : 0u;
+  }
+
+  // Tile-local start of every digit, then stage the tile in digit order.
+  uint32_t total;
+  uint32_t lstart = block_excl_scan<OpSum, kSortThreads>(s_run[tid], s_tmp, &total);
+  s_lstart[tid] = lstart;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
+    if (i < n) {
+      uint32_t d = (key[j] >> shift) & (kRadix - 1);
+      uint32_t lp = s_lstart[d] + rank[j];
+      s_keys[lp] = key[j];
+#pragma unroll
+      for (int p = 0; p < NPAY; ++p) s_pay[p][lp] = pay[p][j];
+    }
+  }
+  __syncthreads();
+
+  // Write out: consecutive threads write consecutive slots of a digit's run.
+  const uint32_t count = (uint32_t)((n - base) < (uint64_t)kSortTile ? (n - base) : kSortTile);
+  for (uint32_t l = tid; l < count; l += kSortThreads) {
+    uint32_t k = s_keys[l];
+    uint32_t d = (k >> shift) & (kRadix - 1);
+    uint32_t g = s_gbase[d] + (l - s_lstart[d]);
+    kout[g] = k;
+#pragma unroll
+    for (int p = 0; p < NPAY; ++p) pout[p][g] = s_pay[p][l];
+  }
+}
+
+// -------------------------------------------------------------- key make ----
+
+__global__ void particle_keys_kernel(const float* __restrict__ xyz, uint64_t n, Grid g,
+                                     uint32_t* __restrict__ key, float* __restrict__ x,
+                                     float* __restrict__ y, float* __restrict__ z) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    key[i] = cell_key(px, py, pz, g);
+    x[i] = px;
+    y[i] = py;
+    z[i] = pz;
+  }
+}
+
+// Spread the low 10 bits of v to every third bit.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void halo_keys_kernel(const float* __restrict__ xyz, uint64_t n,
+                                 uint32_t* __restrict__ key, float* __restrict__ x,
+                                 float* __restrict__ y, float* __restrict__ z,
+                                 uint32_t* __restrict__ row) {
+  const float inv = 1024.0f / 1000.0f;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
+    uint32_t cx = (uint32_t)cell_of(hx, inv, 1023);
+    uint32_t cy = (uint32_t)cell_of(hy, inv, 1023);
+    uint32_t cz = (uint32_t)cell_of(hz, inv, 1023);
+    key[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    x[i] = hx;
+    y[i] = hy;
+    z[i] = hz;
+    row[i] = (uint32_t)i;
+  }
+}
+
+// cell_end pass 1: the last index of every run of equal keys marks its cell.
+__global__ void mark_cell_ends_kernel(const uint32_t* __restrict__ k, uint64_t n,
+                                      uint32_t* __restrict__ end_of) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = k[i];
+    if (i + 1 == n || k[i + 1] != c) end_of[c] = (uint32_t)(i + 1);
+  }
+}
+
+uint32_t grid_blocks(uint64_t n) {
+  uint64_t b = ceil_div(n, 256);
+  return (uint32_t)(b < 262144 ? (b ? b : 1) : 262144);
+}
+
+}  // namespace
+
+hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* ws,
+                             hipStream_t s) {
+  return scan_u32<OpSum, true>(in, out, n, ws, s);
+}
+
+hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* ws,
+                             hipStream_t s) {
+  return scan_u32<OpMax, false>(in, out, n, ws, s);
+}
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_u32_kernel<<<fill_blocks(n), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_u64_kernel<<<fill_blocks(n), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay, uint64_t n,
+                            int key_bits, uint32_t* ws, hipStream_t s, int* result) {
+  *result = 0;
+  if (n <= 1 || key_bits <= 0) return hipSuccess;
+  if (npay != 3 && npay != 4) return hipErrorInvalidValue;
+  const uint64_t tiles = ceil_div(n, kSortTile);
+  if (tiles > 0xffffffffull / kRadix) return hipErrorInvalidValue;
+  uint32_t* hist = ws;
+  uint32_t* scan_ws = ws + (uint64_t)kRadix * tiles;
+  int cur = 0;
+  for (int shift = 0; shift < key_bits; shift += kRadixBits) {
+    const int nxt = cur ^ 1;
+    radix_upsweep_kernel<<<(uint32_t)tiles, kSortThreads, 0, s>>>(keys[cur], n, shift, hist,
+                                                                 (uint32_t)tiles);
+    hipError_t e = exclusive_sum_u32(hist, hist, (uint64_t)kRadix * tiles, scan_ws, s);
+    if (e != hipSuccess) return e;
+    if (npay == 3) {
+      radix_downsweep_kernel<3><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          keys[cur], pay[cur][0], pay[cur][1], pay[cur][2], nullptr, keys[nxt], pay[nxt][0],
+          pay[nxt][1], pay[nxt][2], nullptr, n, shift, hist, (uint32_t)tiles);
+    } else {
+      radix_downsweep_kernel<4><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          keys[cur], pay[cur][0], pay[cur][1], pay[cur][2], pay[cur][3], keys[nxt],
+          pay[nxt][0], pay[nxt][1], pay[nxt][2], pay[nxt][3], n, shift, hist,
+          (uint32_t)tiles);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    cur = nxt;
+  }
+  *result = cur;
+  return hipSuccess;
+}
+
+hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key, float* x,
+                         float* y, float* z, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  particle_keys_kernel<<<grid_blocks(n), 256, 0, s>>>(xyz, n, g, key, x, y, z);
+  return hipGetLastError();
+}
+
+hipError_t halo_keys(const float* xyz, uint64_t n, uint32_t* key, float* x, float* y,
+                     float* z, uint32_t* row, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  halo_keys_kernel<<<grid_blocks(n), 256, 0, s>>>(xyz, n, key, x, y, z, row);
+  return hipGetLastError();
+}
+
+hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n, uint32_t* cell_end,
+                          uint32_t ncells, uint32_t* ws, hipStream_t s) {
+  hipError_t e = fill_u32(cell_end, 0u, ncells, s);
+  if (e != hipSuccess || n == 0) return e;
+  mark_cell_ends_kernel<<<grid_blocks(n), 256, 0, s>>>(sorted_keys, n, cell_end);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return inclusive_max_u32(cell_end, cell_end, ncells, ws, s);
+}
+
+}  // namespace sjcs

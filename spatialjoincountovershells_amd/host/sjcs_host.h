@@ -1,0 +1,120 @@
+/*
+ * sjcs_host.h -- host side of the drop-in astr_rcount replacement: argv
+ * handling, the two input formats, shell thresholds and the text result.
+ * Plain C (gnu99), no HIP: it builds into lib/libsjcs_host.so (so the CPU
+ * tests can drive it) and into bin/sjcs_rcount.
+ */
+#ifndef SJCS_HOST_H
+#define SJCS_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- argv (src/GetOption/GetOption.c) ---- */
+
+/* Last occurrence of `opt` (exact length match) wins; the value is the
+ * argument `offset + 1` places after it.  Returns -1, keeping what earlier
+ * occurrences set, when an occurrence has no value (GetOption.c:7-69). */
+int sjcs_opt_string(int argc, char **argv, const char *opt, int offset, char **dst);
+int sjcs_opt_int(int argc, char **argv, const char *opt, int offset, int *dst);
+/* [*beg, *end) = the arguments after the last `opt` up to the next one that
+ * starts with '-', if there is at least one (GetOption.c:72-93). */
+void sjcs_opt_range(int argc, char **argv, const char *opt, int *beg, int *end);
+int sjcs_opt_flag(int argc, char **argv, const char *opt);
+
+#define SJCS_ERRMSG_LEN 1024
+
+typedef struct sjcs_cli {
+    char *outfile;          /* --output-file, NULL = stdout */
+    int particle_beg, particle_end;
+    int compressed;         /* --compressed */
+    char *halo_file;        /* --halo-file */
+    char *filter_file;      /* --filter-file */
+    int nrads;
+    double minrad, maxrad;
+    char *range_str;
+    int maxbacklogs;        /* --backlog */
+    int nthreads;           /* --thread */
+    int nsetups;            /* --next-setup */
+    int first_nbacklogs;    /* --first-setup a:b */
+    int first_nsetups;
+    char *setup_str;
+    int errflag;
+    char errmsg[SJCS_ERRMSG_LEN];
+} sjcs_cli;
+
+/* Option processing and validation of src/astr_rcount/main.c:492-602,
+ * including its first-error-wins message. */
+void sjcs_cli_parse(int argc, char **argv, sjcs_cli *cli);
+/* The "--execution info--" block (main.c:604-621). */
+void sjcs_cli_print_info(FILE *fp, const sjcs_cli *cli, char **argv);
+/* The usage text (main.c:118-133). */
+void sjcs_cli_print_usage(FILE *fp, const char *argv0);
+
+/* ---- shells (include/support_functions.h:18-34, main.c:645-649) ---- */
+
+void sjcs_host_shell_thresholds(double minrad, double maxrad, int nrads,
+                                float *radiuses, float *radiuses_sq);
+
+/* ---- halos (src/Gadget/BufHalo.c) ---- */
+
+/* In-memory halo record, the reference's Halo (include/Gadget/BufHalo.h:
+ * 26-30): 24 bytes, point at byte 12. */
+typedef struct sjcs_halo {
+    uint64_t id;
+    float mass;
+    float point[3];
+} sjcs_halo;
+
+typedef struct sjcs_halos {
+    sjcs_halo *halo;
+    uint64_t n;
+} sjcs_halos;
+
+#define SJCS_HALO_HEADER_LINES 16
+#define SJCS_HALO_LINE_BUF (1024 * 16)
+
+/* readHalos (BufHalo.c:76-119): 16 header lines, then one halo per line;
+ * a last line without '\n' is dropped.  Parsed in parallel with the same
+ * record boundaries fgets(buf, 16384) would give.  Returns 0, or -1 with
+ * the reference's message in err. */
+int sjcs_read_halos(const char *fname, sjcs_halos *out, int nthreads,
+                    char *err, size_t errlen);
+/* readAndFilterHalos (BufHalo.c:121-214): keeps the halos whose id is in
+ * the filter list (merge join against the sorted list). */
+int sjcs_read_and_filter_halos(const char *fname, const char *filter_fname,
+                               sjcs_halos *out, char *err, size_t errlen);
+/* GetDataFromLineFast (BufHalo.c:218-249) on one NUL-terminated line. */
+int sjcs_parse_halo_line(const char *str, uint64_t *id, float *mass, float point[3]);
+void sjcs_free_halos(sjcs_halos *h);
+
+/* ---- particles (src/Gadget/BufVector3D.c:285-308) ---- */
+
+/* Number of particles in a decoded particle file: size / 12 (trailing
+ * bytes ignored).  -1 when the file cannot be opened. */
+int sjcs_particle_count(const char *fname, uint64_t *n);
+/* Reads the first n particles (3 x FP32 each) into dst. */
+int sjcs_read_particles(const char *fname, float *dst, uint64_t n);
+
+/* ---- result (src/astr_rcount/main.c:371-442) ---- */
+
+/* Writes the text result byte for byte as outputResult does, including its
+ * Total block, which adds halo 0's counts twice (main.c:400,413).
+ * fname NULL = stdout.  Returns 0 or -1. */
+int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *counts,
+                      const float *radiuses_sq, int nrads, int nthreads);
+
+/* milliseconds between two gettimeofday samples, truncated as the
+ * reference's printTimeMillis does (include/time.h:30-39) */
+long sjcs_elapsed_ms(const void *tv_begin, const void *tv_end);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SJCS_HOST_H */
