@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""bench.py -- shell pair-tests/s of the SJCS hot path on one MI355X.
+
+Workload (default): the reference README's search shape, one chunk --
+10M halos x 100M particles x 40 shells (``--radius 0.001:5:40``), halo and
+particle coordinates uniform FP32 in [0, 1000)^3, synthetic (seeded numpy).
+
+A step is one pass of the hot path over the resident particle chunk: cell
+keys, LSD radix sort, cell table, and the count of every halo into the u64
+counts (stored, as for the first particle file).  Inputs are in HBM before
+the timed region starts; the halos are sorted once beforehand, as the
+reference reads its halo file once before its chunk loop.
+
+``value`` = W / (seconds per step), W = exact number of in-box
+(halo image, particle) pair tests of the chunk -- the reference's
+CalcDistance calls -- counted by an untimed pass.
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: the build targets one
+GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import spatialjoincountovershells_amd as sj  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "astr_rcount")
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+CONFIGS = {
+    "c2": dict(workload="uniform 1e8 particles x 1e7 halos, --radius 0.001:5:40, one chunk",
+               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40)),
+    "c1": dict(workload="uniform 1e5 particles x 1e4 halos, --radius 0.001:5:8, one chunk",
+               np=100_000, nh=10_000, radius=(0.001, 5.0, 8)),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(cfg, seed):
+    rng = np.random.default_rng(seed)
+    parts = (rng.random((cfg["np"], 3), dtype=np.float32) * np.float32(1000.0))
+    halos = (rng.random((cfg["nh"], 3), dtype=np.float32) * np.float32(1000.0))
+    # float32 rounding of u*1000 can give exactly 1000.0: keep [0, 1000)
+    np.minimum(parts, np.nextafter(np.float32(1000.0), np.float32(0)), out=parts)
+    np.minimum(halos, np.nextafter(np.float32(1000.0), np.float32(0)), out=halos)
+    return parts, halos
+
+
+def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
+    """Times the reference's astr_rcount (built from its sources into
+    oracle/_ref) on the same particles and a halo sample; returns the
+    cpu_baseline object, or None when the binary is absent."""
+    if not os.path.exists(REF_BIN):
+        return None
+    tmp = tempfile.mkdtemp(prefix="sjcs_cpu_")
+    try:
+        pf = os.path.join(tmp, "part.dat")
+        hf = os.path.join(tmp, "halo.list")
+        parts.tofile(pf)
+        hs = halos[:nsample]
+        with open(hf, "w") as f:
+            for i in range(16):
+                f.write(f"# dummy header {i}\n")
+            # %.9g round-trips FP32 exactly, so strtof gives back these values
+            lines = [f"{i} 0 0 0 0 0 0 0 {x:.9g} {y:.9g} {z:.9g} 0\n"
+                     for i, (x, y, z) in enumerate(hs.tolist())]
+            f.writelines(lines)
+        mn, mx, k = radius
+        cmd = [REF_BIN, "--thread", str(threads), "--next-setup", str(threads),
+               "--backlog", "0", "--particle-files", pf, "--halo-file", hf,
+               "--radius", f"{mn}:{mx}:{k}", "--output-file", os.devnull]
+        t0 = time.time()
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.time() - t0
+        m = re.findall(r">>>> searching time: (\d+) milli seconds", out.stdout)
+        if out.returncode != 0 or not m:
+            log(f"cpu baseline failed rc={out.returncode}: {out.stderr[-300:]}")
+            return None
+        search_s = sum(int(v) for v in m) / 1000.0
+        # exact W of the sample, counted on the GPU with the same floats
+        eng.set_halos(hs)
+        eng.add_particles_device(d_parts, parts.shape[0])
+        w = eng.last_chunk_pair_tests()
+        eng.set_halos(halos)
+        return {"value": w / search_s, "unit": "pair-tests/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"astr_rcount --thread {threads}: all {parts.shape[0]} particles x "
+                          f"{nsample} halos (first {nsample} of the same set), "
+                          f"{mn}:{mx}:{k}; W={w} in {search_s:.3f} s of 'searching time' "
+                          f"(process wall {wall:.1f} s)"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cell-size-milli", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-halos", type=int, default=1_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus}: this build runs on one GPU only "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = CONFIGS[args.config]
+    mn, mx, k = cfg["radius"]
+    _, r2 = sj.shell_thresholds(mn, mx, k)
+    log(f"[bench] generating {cfg['workload']}")
+    parts, halos = make_inputs(cfg, args.seed)
+
+    eng = sj.Engine(0)
+    if args.cell_size_milli:
+        eng.set_option("cell_size_milli", args.cell_size_milli)
+    eng.set_shells(r2)
+    eng.set_halos(halos)
+    d_parts = eng.device_array(parts)
+    n = parts.shape[0]
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        eng.reset_counts()
+        eng.add_particles_device(d_parts, n)
+        eng.sync()
+
+    log(f"[bench] timing {args.steps} steps")
+    phase = np.zeros((args.steps, 3))
+    eng.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        eng.reset_counts()
+        eng.add_particles_device(d_parts, n)
+        phase[s] = eng.last_chunk_timings()  # waits for the step
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1000.0 / args.steps
+
+    w = eng.last_chunk_pair_tests()
+    counted = int(eng.counts().sum())
+    build_ms = float(phase[:, 1].mean())
+    count_ms = float(phase[:, 2].mean())
+    log(f"[bench] W={w} counted={counted} step={ms_step:.3f} ms build={build_ms:.3f} "
+        f"count={count_ms:.3f} ms")
+
+    nh = halos.shape[0]
+    # algorithmic bytes of one count launch: particles read once, halo SoA
+    # (x, y, z, row) read once, counts written once (u64)
+    count_bytes = 12 * n + 16 * nh + 8 * k * nh
+    count_gbs = count_bytes / (count_ms * 1e-3) / 1e9
+    count_tflops = 8 * w / (count_ms * 1e-3) / 1e12
+    # index build: packed xyz in, sorted SoA xyz out, one u32 per cell
+    g = None
+    build_bytes = 24 * n
+    build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
+    dominant = "count" if count_ms >= build_ms else "index_build"
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        traffic = tj.get(args.config, {}).get(dominant)
+    if dominant == "count":
+        roof = {"bound": "hbm", "achieved": round(count_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(count_gbs / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": "count_plain_kernel",
+                "bytes_per_launch": count_bytes}
+    else:
+        roof = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": "index build (keys+radix sort+cell table)",
+                "bytes_per_launch": build_bytes}
+
+    cpu = None
+    if not args.no_cpu_baseline and args.config == "c2":
+        log("[bench] cpu baseline (reference astr_rcount on a halo sample)")
+        cpu = cpu_baseline(parts, halos, cfg["radius"], eng, d_parts,
+                           min(args.cpu_halos, nh), args.cpu_threads)
+
+    eng.device_free(d_parts)
+    eng.close()
+    value = w / (ms_step * 1e-3)
+    out = {
+        "metric": "shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)",
+        "value": value,
+        "unit": "pair-tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "particles": n, "halos": nh, "shells": k,
+                   "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
+                   "pairs_counted": counted, "parallelism": "single GPU"},
+        "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4)},
+        "count_kernel": {"pair_tests_per_s": w / (count_ms * 1e-3),
+                         "valu_tflops_8_per_test": round(count_tflops, 3),
+                         "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
+                         "hbm_gbs_algorithmic": round(count_gbs, 1)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,12 @@ int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
  * [0] upload, [1] index build (key + sort + cell table), [2] count. */
 int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);
 
+/* Device memory helpers for callers without a HIP runtime of their own
+ * (benchmarks, bindings).  No reference counterpart. */
+int sjcs_device_alloc(sjcs_engine *e, size_t bytes, void **d_ptr);
+int sjcs_device_free(sjcs_engine *e, void *d_ptr);
+int sjcs_copy_to_device(sjcs_engine *e, void *d_dst, const void *src, size_t bytes);
+
 /* ---- index: the ArraySTR lifecycle, one device-resident particle index ---- */
 
 int sjcs_index_create(sjcs_engine *e, uint64_t ndata, sjcs_index **out);

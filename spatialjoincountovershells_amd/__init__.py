@@ -44,6 +44,7 @@ EXPORTED = (
     "sjcs_set_option", "sjcs_last_chunk_timings",
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
+    "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device",
 )
 
 
@@ -89,6 +90,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_index_build": (i32, [vp]),
         "sjcs_index_count": (i32, [vp, vp, sz, u64, vp, i32, vp]),
         "sjcs_index_destroy": (None, [vp]),
+        "sjcs_device_alloc": (i32, [vp, sz, pp]),
+        "sjcs_device_free": (i32, [vp, vp]),
+        "sjcs_copy_to_device": (i32, [vp, vp, vp, sz]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -190,6 +194,18 @@ class Engine:
         ms = (ctypes.c_float * 3)()
         self._check(self._lib.sjcs_last_chunk_timings(self._h, ms))
         return tuple(ms)
+
+    def device_array(self, host: np.ndarray) -> int:
+        """Copies a host array into new device memory; returns the device
+        address (free it with :meth:`device_free`)."""
+        host = np.ascontiguousarray(host)
+        ptr = ctypes.c_void_p()
+        self._check(self._lib.sjcs_device_alloc(self._h, host.nbytes, ctypes.byref(ptr)))
+        self._check(self._lib.sjcs_copy_to_device(self._h, ptr, _ptr(host), host.nbytes))
+        return ptr.value
+
+    def device_free(self, ptr: int):
+        self._check(self._lib.sjcs_device_free(self._h, ctypes.c_void_p(ptr)))
 
     def set_option(self, key: str, value: int):
         self._check(self._lib.sjcs_set_option(self._h, key.encode(), int(value)))

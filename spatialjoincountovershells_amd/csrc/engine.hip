@@ -592,6 +592,33 @@ int sjcs_last_chunk_timings(sjcs_engine* e, float ms[3]) {
   return SJCS_OK;
 }
 
+int sjcs_device_alloc(sjcs_engine* e, size_t bytes, void** d_ptr) {
+  if (!e || !d_ptr) return SJCS_ERR_INVALID;
+  *d_ptr = nullptr;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipMalloc(d_ptr, bytes ? bytes : 16), "hipMalloc");
+  return SJCS_OK;
+}
+
+int sjcs_device_free(sjcs_engine* e, void* d_ptr) {
+  if (!e) return SJCS_ERR_INVALID;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  if (d_ptr) HIPCALL(e, hipFree(d_ptr), "hipFree");
+  return SJCS_OK;
+}
+
+int sjcs_copy_to_device(sjcs_engine* e, void* d_dst, const void* src, size_t bytes) {
+  if (!e || (!d_dst && bytes) || (!src && bytes)) return SJCS_ERR_INVALID;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (bytes == 0) return SJCS_OK;
+  HIPCALL(e, hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  return SJCS_OK;
+}
+
 // ---- index API ----
 
 int sjcs_index_create(sjcs_engine* e, uint64_t ndata, sjcs_index** out) {

@@ -1,0 +1,261 @@
+"""Helpers shared by the tests: the oracle's ctypes bindings, readers of the
+golden fixtures, and an independent restatement of the reference's halo
+line parse (src/Gadget/BufHalo.c:218-249) that calls the C library's own
+strtoul/strtof through ctypes, so the oracle tests do not depend on the
+product's parser."""
+from __future__ import annotations
+
+import ctypes
+import gzip
+import json
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_lib", "libsjcs_oracle.so")
+DPPS_PROBE = os.path.join(ORACLE_DIR, "_lib", "dpps_probe")
+
+_libc = ctypes.CDLL("libc.so.6")
+_libc.strtoul.restype = ctypes.c_ulong
+_libc.strtoul.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int]
+_libc.strtof.restype = ctypes.c_float
+_libc.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+_libc.strtol.restype = ctypes.c_long
+_libc.strtol.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int]
+
+
+def manifest():
+    with open(os.path.join(GOLD, "manifest.json")) as f:
+        return json.load(f)
+
+
+# ------------------------------------------------------------------ oracle --
+
+_oracle = None
+
+
+def oracle():
+    """The oracle library, built with gcc on first use if needed."""
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not os.path.exists(ORACLE_LIB) or not os.path.exists(DPPS_PROBE):
+        subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+    lib = ctypes.CDLL(ORACLE_LIB)
+    vp = ctypes.c_void_p
+    lib.oracle_count.restype = ctypes.c_uint64
+    lib.oracle_count.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_size_t, ctypes.c_uint64,
+                                 vp, ctypes.c_int, vp, ctypes.c_int]
+    lib.oracle_shell_thresholds.restype = None
+    lib.oracle_shell_thresholds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            vp, vp]
+    lib.oracle_shell_thresholds_nofma.restype = None
+    lib.oracle_shell_thresholds_nofma.argtypes = lib.oracle_shell_thresholds.argtypes
+    lib.oracle_distance.restype = ctypes.c_float
+    lib.oracle_distance.argtypes = [vp, vp]
+    lib.oracle_shell_of.restype = ctypes.c_int
+    lib.oracle_shell_of.argtypes = [ctypes.c_float, vp, ctypes.c_int]
+    lib.oracle_images.restype = ctypes.c_int
+    lib.oracle_images.argtypes = [vp, ctypes.c_float, vp]
+    lib.oracle_box_radius.restype = ctypes.c_float
+    lib.oracle_box_radius.argtypes = [vp, ctypes.c_int]
+    _oracle = lib
+    return lib
+
+
+def oracle_thresholds(minrad, maxrad, k):
+    r = np.zeros(k, np.float32)
+    r2 = np.zeros(k, np.float32)
+    oracle().oracle_shell_thresholds(minrad, maxrad, k, r.ctypes.data, r2.ctypes.data)
+    return r, r2
+
+
+def oracle_count(particles, halos, r2, threads=8):
+    """(counts [nh][K] uint64, W) by brute force."""
+    p = np.ascontiguousarray(particles, np.float32).reshape(-1, 3)
+    h = np.ascontiguousarray(halos, np.float32).reshape(-1, 3)
+    r2 = np.ascontiguousarray(r2, np.float32)
+    c = np.zeros((h.shape[0], r2.size), np.uint64)
+    w = oracle().oracle_count(p.ctypes.data, p.shape[0], h.ctypes.data, 12, h.shape[0],
+                              r2.ctypes.data, r2.size, c.ctypes.data, threads)
+    return c, w
+
+
+def parse_radius(spec):
+    """`--radius` as astr_rcount reads it (src/astr_rcount/main.c:506-533)."""
+    mn = float(spec.split(":")[0])
+    rest = spec.split(":", 1)[1]
+    mx = float(_strtod_prefix(rest))
+    k = int(_libc.strtol(spec.rsplit(":", 1)[1].encode(), None, 10))
+    if mn > mx:
+        mn, mx = mx, mn
+    return mn, mx, k
+
+
+def _strtod_prefix(s):
+    strtod = _libc.strtod
+    strtod.restype = ctypes.c_double
+    strtod.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    return strtod(s.encode(), None)
+
+
+# ----------------------------------------------------------------- inputs --
+
+def read_particles(path):
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = raw.size // 12
+    return raw[: n * 12].view(np.float32).reshape(n, 3)
+
+
+def _fgets_records(data: bytes, cap=16383):
+    """Yields (record, at_eof) as fgets(buf, 16384) + feof would."""
+    pos = 0
+    size = len(data)
+    while True:
+        left = size - pos
+        lim = min(left, cap)
+        nl = data.find(b"\n", pos, pos + lim)
+        if nl >= 0:
+            yield data[pos:nl + 1], False
+            pos = nl + 1
+        else:
+            yield data[pos:pos + lim], left < cap
+            if left < cap:
+                return
+            pos += lim
+
+
+def parse_halo_line(rec: bytes):
+    """GetDataFromLineFast with libc strtoul/strtof; None when a space is
+    missing.  The record is cut at its first NUL, as a C string is."""
+    s = rec.split(b"\0", 1)[0]
+    hid = _libc.strtoul(s, None, 10)
+    pos = 0
+    for _ in range(2):
+        i = s.find(b" ", pos)
+        if i < 0:
+            return None
+        pos = i + 1
+    mass = _libc.strtof(s[pos:], None)
+    for _ in range(6):
+        i = s.find(b" ", pos)
+        if i < 0:
+            return None
+        pos = i + 1
+    xyz = [_libc.strtof(s[pos:], None)]
+    for _ in range(2):
+        i = s.find(b" ", pos)
+        if i < 0:
+            return None
+        pos = i + 1
+        xyz.append(_libc.strtof(s[pos:], None))
+    return hid, mass, xyz
+
+
+def read_halos(path):
+    """(ids uint64, points float32 [n][3]) as readHalos reads the file."""
+    with open(path, "rb") as f:
+        data = f.read()
+    recs = _fgets_records(data)
+    for _ in range(16):
+        rec, eof = next(recs)
+        if eof:
+            raise ValueError("no headers")
+    ids, pts = [], []
+    for rec, eof in recs:
+        if eof:
+            break
+        r = parse_halo_line(rec)
+        if r is None:
+            raise ValueError("bad line")
+        ids.append(r[0])
+        pts.append(r[2])
+    return np.array(ids, np.uint64), np.array(pts, np.float32).reshape(-1, 3)
+
+
+def filter_halos(ids, pts, filter_path):
+    """readAndFilterHalos' merge join (src/Gadget/BufHalo.c:121-214)."""
+    with open(filter_path, "rb") as f:
+        data = f.read()
+    fl = [np.uint64(_libc.strtol(rec.split(b"\0", 1)[0], None, 10) & 0xFFFFFFFFFFFFFFFF)
+          for rec, eof in _fgets_records(data) if rec]
+    fl = sorted(int(v) for v in fl)
+    keep = []
+    fr = 0
+    for i, hid in enumerate(ids.tolist()):
+        if hid < fl[fr]:
+            continue
+        while hid > fl[fr]:
+            fr += 1
+            if fr >= len(fl):
+                break
+        if fr >= len(fl):
+            break
+        if hid == fl[fr]:
+            keep.append(i)
+    return ids[keep], pts[keep]
+
+
+def filter_halos_nfilters_zero(ids, pts, filter_path):
+    """What the reference actually does: ConstructFilterIdList
+    (src/Gadget/BufHalo.c:251-300) never stores the id count, so
+    fil->nfilters is whatever the fresh heap block holds -- 0 in practice.
+    The list is then not sorted (RadixSort of 0 ids) and the merge join
+    ends at the first halo whose id exceeds the file's first id."""
+    with open(filter_path, "rb") as f:
+        first = _libc.strtol(f.readline(), None, 10) & 0xFFFFFFFFFFFFFFFF
+    keep = []
+    for i, hid in enumerate(ids.tolist()):
+        if hid < first:
+            continue
+        if hid > first:
+            break
+        keep.append(i)
+    return ids[keep], pts[keep]
+
+
+def case_inputs(case, filter_as_reference_runs=False):
+    """Particle chunks, halo ids and points of a golden case."""
+    chunks = [read_particles(os.path.join(GOLD, p)) for p in case["particle_files"]]
+    ids, pts = read_halos(os.path.join(GOLD, case["halo_file"]))
+    if case.get("filter_file"):
+        fn = filter_halos_nfilters_zero if filter_as_reference_runs else filter_halos
+        ids, pts = fn(ids, pts, os.path.join(GOLD, case["filter_file"]))
+    return chunks, ids, pts
+
+
+# ---------------------------------------------------------------- results --
+
+def read_expected_text(name):
+    with gzip.open(os.path.join(GOLD, name), "rb") as f:
+        return f.read()
+
+
+def parse_result(text: bytes, k: int):
+    """(ids, counts [n][K], total [K], labels [K]) from a result file."""
+    lines = text.decode().split("\n")
+    ids, rows = [], []
+    i = 0
+    labels = None
+    while i < len(lines) and lines[i].startswith("id "):
+        ids.append(int(lines[i][3:]))
+        row, lab = [], []
+        for j in range(k):
+            head, val = lines[i + 1 + j].rsplit(": ", 1)
+            row.append(int(val))
+            lab.append(head)
+        labels = lab
+        rows.append(row)
+        i += k + 1
+    assert lines[i] == "" and lines[i + 1] == "Total", lines[i:i + 2]
+    total, tlab = [], []
+    for j in range(k):
+        head, val = lines[i + 2 + j].rsplit(": ", 1)
+        total.append(int(val))
+        tlab.append(head)
+    return (np.array(ids, np.uint64), np.array(rows, np.uint64).reshape(-1, k),
+            np.array(total, np.uint64), tlab)

@@ -1,0 +1,182 @@
+"""Host side of the drop-in CLI, CPU only: argv errors (stderr, exit status,
+stdout) against the reference's, the halo and filter readers, shell
+thresholds, and the result writer byte for byte -- all without a GPU."""
+import ctypes
+import gzip
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sjcs_testlib as t
+
+MANIFEST = t.manifest()
+CASES = MANIFEST["cases"]
+
+
+class Halo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("mass", ctypes.c_float),
+                ("point", ctypes.c_float * 3)]
+
+
+class Halos(ctypes.Structure):
+    _fields_ = [("halo", ctypes.POINTER(Halo)), ("n", ctypes.c_uint64)]
+
+
+@pytest.fixture(scope="module")
+def host(built):
+    lib = ctypes.CDLL(built.HOST_LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.sjcs_read_halos.argtypes = [ctypes.c_char_p, ctypes.POINTER(Halos), ctypes.c_int,
+                                    ctypes.c_char_p, ctypes.c_size_t]
+    lib.sjcs_read_and_filter_halos.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                               ctypes.POINTER(Halos), ctypes.c_char_p,
+                                               ctypes.c_size_t]
+    lib.sjcs_free_halos.argtypes = [ctypes.POINTER(Halos)]
+    lib.sjcs_write_result.argtypes = [ctypes.c_char_p, ctypes.POINTER(Halos), vp, vp,
+                                      ctypes.c_int, ctypes.c_int]
+    lib.sjcs_host_shell_thresholds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               vp, vp]
+    lib.sjcs_particle_count.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+    lib.sjcs_parse_halo_line.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float)]
+    return lib
+
+
+def read_with_host(host, path, nthreads=4, filter_path=None):
+    hs = Halos()
+    err = ctypes.create_string_buffer(4096)
+    if filter_path is None:
+        rc = host.sjcs_read_halos(path.encode(), ctypes.byref(hs), nthreads, err, 4096)
+    else:
+        rc = host.sjcs_read_and_filter_halos(path.encode(), filter_path.encode(),
+                                             ctypes.byref(hs), err, 4096)
+    if rc != 0:
+        return None, err.value.decode()
+    ids = np.array([hs.halo[i].id for i in range(hs.n)], np.uint64)
+    pts = np.array([list(hs.halo[i].point) for i in range(hs.n)], np.float32).reshape(-1, 3)
+    return (hs, ids, pts), ""
+
+
+@pytest.mark.parametrize("i", range(len(MANIFEST["option_errors"])))
+def test_option_errors_match_reference(built, i, tmp_path):
+    case = MANIFEST["option_errors"][i]
+    r = subprocess.run([built.CLI_PATH] + case["args"], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=60)
+    assert r.returncode == case["returncode"]
+    assert r.stderr.replace(built.CLI_PATH, "ARGV0") == case["stderr"]
+    assert r.stdout == case["stdout"]
+
+
+@pytest.mark.parametrize("nthreads", [1, 3, 16])
+@pytest.mark.parametrize("name", ["uni_halos.list", "fmt_halos.list", "tie_halos.list",
+                                  "tdpb_halos.list", "rtree_halos.list"])
+def test_halo_reader_matches_libc_restatement(host, name, nthreads):
+    path = os.path.join(t.GOLD, name)
+    (hs, ids, pts), err = read_with_host(host, path, nthreads)
+    want_ids, want_pts = t.read_halos(path)
+    assert np.array_equal(ids, want_ids)
+    assert np.array_equal(pts.view(np.uint32), want_pts.view(np.uint32))
+    host.sjcs_free_halos(ctypes.byref(hs))
+
+
+def test_halo_reader_edge_records(host, tmp_path):
+    """fgets record rules: a line longer than 16383 bytes is several records,
+    a last line without '\\n' is dropped, a NUL ends the parsed string."""
+    long_tail = "9" * 20000
+    lines = [f"# h {i}" for i in range(16)]
+    body = ["1 0 5 0 0 0 0 0 1.5 2.5 3.5 0",
+            "2 0 5 0 0 0 0 0 4 5 6 " + long_tail,
+            "3 0 5 0 0 0 0 0 7\x008 9 10 11 12"]
+    p = tmp_path / "h.list"
+    p.write_bytes(("\n".join(lines + body) + "\n4 0 5 0 0 0 0 0 1 1 1").encode())
+    res, err = read_with_host(host, str(p))
+    # the tail of line 2 is its own record with no spaces: the reference fails
+    assert res is None and "may be incorrect" in err
+    try:
+        t.read_halos(str(p))
+        raise AssertionError("restatement accepted it")
+    except ValueError:
+        pass
+    p.write_bytes(("\n".join(lines + body[:1]) + "\n4 0 5 0 0 0 0 0 1 1 1").encode())
+    (hs, ids, pts), _ = read_with_host(host, str(p))
+    assert ids.tolist() == [1] and pts.tolist() == [[1.5, 2.5, 3.5]]
+    host.sjcs_free_halos(ctypes.byref(hs))
+
+
+def test_halo_reader_errors(host, tmp_path):
+    res, err = read_with_host(host, str(tmp_path / "missing.list"))
+    assert res is None and err.startswith("ERROR: file ") and "not found in readHalos()" in err
+    p = tmp_path / "short.list"
+    p.write_text("# only\n# two headers\n")
+    res, err = read_with_host(host, str(p))
+    assert res is None and "does not contain headers in readHalos()" in err
+
+
+def test_filter_reader_intended_semantics(host):
+    path = os.path.join(t.GOLD, "uni_halos.list")
+    fpath = os.path.join(t.GOLD, "uni_filter.txt")
+    (hs, ids, pts), _ = read_with_host(host, path, filter_path=fpath)
+    ids0, pts0 = t.read_halos(path)
+    want_ids, want_pts = t.filter_halos(ids0, pts0, fpath)
+    assert np.array_equal(ids, want_ids) and np.array_equal(pts, want_pts)
+    # every listed id below 2000 is kept (ids are 0..1999, sorted, unique)
+    listed = {int(x) for x in open(fpath).read().split()}
+    assert set(ids.tolist()) == {i for i in listed if i < 2000}
+    host.sjcs_free_halos(ctypes.byref(hs))
+
+
+@pytest.mark.parametrize("spec", ["0.001:5:8", "0.001:5:40", "0.01:20:64", "5:0.001:8",
+                                  "0.5:100:16", "3:7:1", "0.25:2:7", "1e-6:900:300"])
+def test_thresholds_bitwise_equal_oracle(host, spec):
+    mn, mx, k = t.parse_radius(spec)
+    r = np.zeros(k, np.float32)
+    r2 = np.zeros(k, np.float32)
+    host.sjcs_host_shell_thresholds(mn, mx, k, r.ctypes.data, r2.ctypes.data)
+    _, want = t.oracle_thresholds(mn, mx, k)
+    assert np.array_equal(r2.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_result_writer_byte_identical(host, case, tmp_path):
+    """Counts from the oracle (pinned to the reference) written by the host
+    writer give the reference's result file byte for byte."""
+    if case.get("filter_file"):
+        pytest.skip("the reference's filter reads an uninitialised count; see test_cli")
+    mn, mx, k = t.parse_radius(case["radius"])
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    chunks, ids, pts = t.case_inputs(case)
+    tot = np.zeros((pts.shape[0], k), np.uint64)
+    for ch in chunks:
+        tot += t.oracle_count(ch, pts, r2)[0]
+    arr = (Halo * max(1, len(ids)))()
+    for i in range(len(ids)):
+        arr[i].id = int(ids[i])
+        arr[i].point[:] = pts[i].tolist()
+    hs = Halos(ctypes.cast(arr, ctypes.POINTER(Halo)), len(ids))
+    out = tmp_path / "out.txt"
+    for nthreads in (1, 5):
+        rc = host.sjcs_write_result(str(out).encode(), ctypes.byref(hs), tot.ctypes.data,
+                                    r2.ctypes.data, k, nthreads)
+        assert rc == 0
+        assert out.read_bytes() == t.read_expected_text(case["expected"])
+
+
+def test_particle_count_ignores_trailing_bytes(host, tmp_path):
+    p = tmp_path / "p.dat"
+    p.write_bytes(np.arange(30, dtype=np.float32).tobytes() + b"\x01\x02\x03\x04\x05")
+    n = ctypes.c_uint64()
+    assert host.sjcs_particle_count(str(p).encode(), ctypes.byref(n)) == 0
+    assert n.value == 10
+    assert host.sjcs_particle_count(str(tmp_path / "none").encode(), ctypes.byref(n)) == -1
+
+
+def test_cli_missing_files_fail_cleanly(built, tmp_path):
+    r = subprocess.run([built.CLI_PATH, "--thread", "2", "--next-setup", "1",
+                        "--particle-files", "nope.dat", "--halo-file", "nope.list",
+                        "--radius", "0.1:5:4"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=60)
+    assert r.returncode == 1
+    assert "ERROR: file nope.list not found in readHalos()." in r.stderr

@@ -1,0 +1,145 @@
+"""Pins the oracle (oracle/sjcs_oracle.c), CPU only.
+
+Against: the golden vectors the compiled reference produced
+(tests/golden/), the hand-derived periodic known answer of the reference's
+TestData4PB, the SSE4.1 DPPS instruction itself, and closed forms on
+integer lattices -- each chosen so that a plausible slip (a dropped periodic
+image, a shifted sign, <= for <, an FMA in the distance, a swapped axis)
+fails at least one of them.
+"""
+import itertools
+import subprocess
+
+import numpy as np
+import pytest
+
+import sjcs_testlib as t
+
+CASES = t.manifest()["cases"]
+
+
+def oracle_case_counts(case, as_reference_runs=True):
+    mn, mx, k = t.parse_radius(case["radius"])
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    chunks, ids, pts = t.case_inputs(case, filter_as_reference_runs=as_reference_runs)
+    tot = np.zeros((pts.shape[0], k), np.uint64)
+    for ch in chunks:
+        c, _ = t.oracle_count(ch, pts, r2)
+        tot += c
+    return ids, tot, k
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_oracle_equals_reference_golden(case):
+    ids, tot, k = oracle_case_counts(case)
+    eids, ecnt, etot, _ = t.parse_result(t.read_expected_text(case["expected"]), k)
+    assert np.array_equal(eids, ids)
+    assert np.array_equal(ecnt, tot)
+    # Total = sum over halos + halo 0 once more (main.c:400,413)
+    want_total = tot.sum(axis=0, dtype=np.uint64) + (tot[0] if tot.shape[0] else 0)
+    assert np.array_equal(etot, want_total)
+
+
+def test_tdpb_known_answer():
+    """TestData4PB with --radius 0.001:5:8, derived by hand: each halo sees
+    the 8 corner particles through its images at d^2 = 0, 1 (x3), 2 (x3), 3."""
+    case = [c for c in CASES if c["name"] == "tdpb_8"][0]
+    ids, tot, k = oracle_case_counts(case)
+    per_halo = np.array([1, 0, 0, 0, 0, 0, 6, 1], np.uint64)
+    assert np.array_equal(tot, np.stack([per_halo, per_halo]))
+    _, _, etot, _ = t.parse_result(t.read_expected_text(case["expected"]), k)
+    assert np.array_equal(etot, 3 * per_halo)
+
+
+def test_dpps_equivalence_on_this_cpu():
+    t.oracle()
+    out = subprocess.run([t.DPPS_PROBE, "3000000", "12345"], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("dpps ok")
+
+
+def test_thresholds_match_reference_labels():
+    """The %f radius labels of every golden result equal the oracle's
+    thresholds printed the same way (outputResult: sqrt of r^2)."""
+    for case in CASES:
+        mn, mx, k = t.parse_radius(case["radius"])
+        _, r2 = t.oracle_thresholds(mn, mx, k)
+        *_, labels = t.parse_result(t.read_expected_text(case["expected"]), k)
+        rad = [np.float32(np.sqrt(np.float64(v))) for v in r2]
+        want = ["  [0.0 ~ %f]" % rad[0]] + ["  [%f ~ %f]" % (rad[i - 1], rad[i])
+                                          for i in range(1, k)]
+        assert labels == want, case["name"]
+
+
+def test_fma_does_not_move_config_thresholds():
+    """Logspace's a + i*c is contracted to an FMA by gcc -O2 on an FMA host;
+    for every radius spec of the workloads both forms agree bit for bit, so
+    the FMA is kept for exactness but cannot be observed in these counts."""
+    for mn, mx, k in [(0.001, 5, 8), (0.001, 5, 40), (0.01, 20, 64), (0.5, 100, 16)]:
+        r = np.zeros(k, np.float32)
+        r2a = np.zeros(k, np.float32)
+        r2b = np.zeros(k, np.float32)
+        t.oracle().oracle_shell_thresholds(mn, mx, k, r.ctypes.data, r2a.ctypes.data)
+        t.oracle().oracle_shell_thresholds_nofma(mn, mx, k, r.ctypes.data, r2b.ctypes.data)
+        assert np.array_equal(r2a.view(np.uint32), r2b.view(np.uint32))
+
+
+def lattice_counts(a_num, a_den, n, r2_units, center_offset):
+    """Closed form: particles on the dyadic lattice c + a*(i, j, k), |i|,|j|,|k|
+    <= n, a = a_num / a_den; all distances and box edges are exact in FP32, so
+    the count in shell s is the number of integer triples with
+    r2_units[s-1] <= i^2 + j^2 + k^2 < r2_units[s] and |i|,|j|,|k| <= rbox/a."""
+    rbox_units = int(np.floor(np.sqrt(r2_units[-1])))
+    counts = [0] * len(r2_units)
+    for i, j, k in itertools.product(range(-n, n + 1), repeat=3):
+        if max(abs(i), abs(j), abs(k)) > rbox_units:
+            continue
+        q = i * i + j * j + k * k
+        for s, lim in enumerate(r2_units):
+            if q < lim:
+                counts[s] += 1
+                break
+    return counts
+
+
+@pytest.mark.parametrize("r2_units", [[1, 2, 3, 4], [2, 5, 9, 16], [1, 9, 25], [3, 8, 12, 25]])
+def test_lattice_closed_form(r2_units):
+    a = 0.25
+    n = 7
+    c = np.array([300.0, 400.0, 500.0], np.float32)
+    g = np.arange(-n, n + 1, dtype=np.float32) * np.float32(a)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + c
+    r2 = np.array(r2_units, np.float32) * np.float32(a * a)
+    got, _ = t.oracle_count(pts.astype(np.float32), c[None, :], r2)
+    assert got[0].tolist() == lattice_counts(1, 4, n, r2_units, c)
+
+
+def test_periodic_images_closed_form():
+    """A halo at a box corner sees particles across all faces: with a
+    lattice around (0,0,0) wrapped into [0,1000), every image is needed and
+    the counts equal the un-wrapped lattice's closed form."""
+    a = 0.5
+    n = 4
+    g = np.arange(-n, n + 1, dtype=np.float32) * np.float32(a)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    wrapped = np.where(pts < 0, pts + np.float32(1000.0), pts).astype(np.float32)
+    r2_units = [1, 3, 6, 13]
+    r2 = np.array(r2_units, np.float32) * np.float32(a * a)
+    got, _ = t.oracle_count(wrapped, np.zeros((1, 3), np.float32), r2)
+    assert got[0].tolist() == lattice_counts(1, 2, n, r2_units, None)
+
+
+def test_images_order_and_shift_rule():
+    """Image centres of a corner halo: all 8, +1000 on low-side axes, -1000
+    on high-side axes; the low side wins when both hold (r_box > 500)."""
+    o = t.oracle()
+    cent = np.zeros((8, 3), np.float32)
+    n = o.oracle_images(np.array([1.0, 999.0, 500.0], np.float32).ctypes.data,
+                        np.float32(5.0), cent.ctypes.data)
+    got = {tuple(map(float, r)) for r in cent[:n]}
+    assert got == {(1.0, 999.0, 500.0), (1001.0, 999.0, 500.0), (1.0, -1.0, 500.0),
+                   (1001.0, -1.0, 500.0)}
+    n = o.oracle_images(np.array([500.0, 500.0, 500.0], np.float32).ctypes.data,
+                        np.float32(600.0), cent.ctypes.data)
+    assert n == 8 and cent[:n].max(axis=0).tolist() == [1500.0, 1500.0, 1500.0]
+    assert cent[:n].min(axis=0).tolist() == [500.0, 500.0, 500.0]
