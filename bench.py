@@ -117,6 +117,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cell-size-milli", type=int, default=0)
+    ap.add_argument("--lanes-per-halo", type=int, default=0)
+    ap.add_argument("--plain", action="store_true",
+                    help="one halo per wavefront count kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-halos", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -135,6 +138,10 @@ def main():
     eng = sj.Engine(0)
     if args.cell_size_milli:
         eng.set_option("cell_size_milli", args.cell_size_milli)
+    if args.lanes_per_halo:
+        eng.set_option("lanes_per_halo", args.lanes_per_halo)
+    if args.plain:
+        eng.set_option("kernel", 1)
     eng.set_shells(r2)
     eng.set_halos(halos)
     d_parts = eng.device_array(parts)

@@ -103,8 +103,11 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
 
 /* Tuning knobs (no reference counterpart).  Keys:
  *   "cell_size_milli"   cell edge in 1/1000 box units, 0 = automatic
- *   "kernel"            0 = automatic, 1 = one halo per wavefront (plain),
- *                       2 = batched work-queue kernel
+ *   "kernel"            0 = automatic, 1 = one halo per wavefront (plain)
+ *   "lanes_per_halo"    16 or 64 lanes of a wavefront per halo, 0 =
+ *                       automatic (16 up to 256 shells)
+ *   "segment"           particles of a cell row one lane walks: 4 or 8,
+ *                       0 = automatic
  * Returns SJCS_ERR_INVALID for an unknown key or value. */
 int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
 

@@ -24,6 +24,7 @@ constexpr float kBoxHi = 1000.0f;
 // run of x cells of a row is one contiguous index range.
 struct Grid {
   float inv_cell;  // G / 1000
+  float cell;      // 1000 / G
   int g;           // cells per axis
   uint32_t ncells; // g^3
 };
@@ -62,21 +63,30 @@ __device__ __forceinline__ uint32_t cell_key(float x, float y, float z,
 // Shell lookup tables for the first-match rule.  When r2 is non-decreasing,
 // NaN-free and sign-bit-free, the FP32 bit pattern of a non-negative distance
 // is monotone in its value, so a table indexed by (bits(d) - bits(r2[0])) >>
-// shift gives a lower bound on the first matching shell, and a short upward
-// scan finishes it exactly.
-struct Shells {
-  const float* r2;       // device, nrads
-  const uint16_t* lut;   // device, nlut (sorted case only)
-  int nrads;
-  int sorted;            // 1: lut path valid; 0: plain first-match scan
-  uint32_t lut_base;     // bits of r2[0]
-  int lut_shift;
-  int nlut;
-  float rbox;            // (float)sqrt((double)r2[nrads-1]), from the host
-  float r2max;           // r2[nrads-1]
+// shift gives a lower bound k on the first matching shell.  Each entry also
+// carries r2[k]; when no bucket holds two thresholds (`one_step`), the shell
+// is k + (d >= r2[k]), else a short upward scan finishes it.  The base is 0
+// (bucket = bits >> shift) whenever that table is fine enough.
+struct ShellEntry {
+  uint32_t off;  // 4 * k, k = first shell that can match a distance of this bucket
+  float t;       // r2[k]
 };
 
-constexpr int kLutSize = 2048;
+struct Shells {
+  const float* r2;          // device, nrads
+  const ShellEntry* lut;    // device, nlut (sorted case only)
+  int nrads;
+  int sorted;               // 1: lut path valid; 0: plain first-match scan
+  int one_step;             // 1: every bucket holds at most one threshold
+  uint32_t lut_base;        // bits of r2[0]
+  int lut_shift;
+  int nlut;
+  float rbox;               // (float)sqrt((double)r2[nrads-1]), from the host
+  float r2max;              // r2[nrads-1]
+  float r2max_hi;           // r2max * (1 + 1e-5): margin of the sphere cut
+};
+
+constexpr int kLutSize = 1024;
 constexpr int kMaxLdsShells = 1024;  // larger K counts straight into HBM
 
 // Device buffers of one indexed particle chunk.

@@ -4,13 +4,17 @@
 // walk (src/objects/ArraySTR/MultiRangeCount.c:13-115), driven per halo by
 // threadJoin (src/astr_rcount/main.c:795-826).
 //
-// Plain kernel: one wavefront per halo.  For the halo and each periodic
-// image of it, the query box's cell range is cut into (y, z) rows whose x
-// runs are contiguous in the sorted particle arrays; the wave scans the row
-// lengths and its 64 lanes stride over the concatenated candidates with
-// coalesced SoA loads.  Every candidate gets the reference's box test,
-// FP32 distance and first-match shell, and lands in a per-wave LDS
-// histogram that is flushed to the u64 counts when the halo is done.
+// One halo per GROUP lanes of a wavefront (GROUP 64: the plain
+// one-halo-per-wavefront kernel; 16: four halos in flight per wavefront).
+// For the halo and each of its periodic images the query box's cell range is
+// cut into (y, z) rows, whose x runs are contiguous in the sorted particle
+// arrays.  Row r of a batch belongs to lane r of the group.  Rows are cut
+// into segments of SEG consecutive particles; segments are dealt out GROUP
+// at a time, a lane finds the row of its segment from a mark + max-scan over
+// the group (one LDS round trip), then walks the segment's particles with
+// immediate-offset buffer loads.  Each candidate gets the reference's box
+// test, FP32 distance and first-match shell, and lands in the group's LDS
+// histogram, flushed to the u64 counts when the halo is done.
 #include "count.h"
 
 #pragma clang fp contract(off)
@@ -18,37 +22,91 @@
 namespace sjcs {
 namespace {
 
+constexpr int kShellScan = 0;     // any thresholds: first match by linear scan
+constexpr int kShellTable = 1;    // sorted: table lower bound + upward scan
+constexpr int kShellOneStep = 2;  // sorted, one threshold per bucket at most
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, int lane) {
+// DPP row_shr:n within each row of 16 lanes; lanes whose source falls before
+// the row start read 0.
+template <int N>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xf, 0xf, false);
+}
+
+// Inclusive sum / max over the GROUP lanes of a group (values >= 0).
+template <int GROUP>
+__device__ __forceinline__ uint32_t grp_scan_add(uint32_t v, int gl) {
+  if constexpr (GROUP == 16) {
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    v += row_shr<8>(v);
+  } else {
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
+    for (int off = 1; off < GROUP; off <<= 1) {
+      uint32_t t = __shfl_up(v, off, GROUP);
+      if (gl >= off) v += t;
+    }
   }
   return v;
 }
 
-// First k with d < r2[k], or -1 (src/objects/ArraySTR/MultiRangeCount.c:107-113).
-template <bool kSorted>
-__device__ __forceinline__ int shell_of(float d, const float* r2, const uint16_t* lut,
-                                        const Shells& sh) {
-  if (kSorted) {
-    if (!(d < sh.r2max)) return -1;  // also rejects NaN
-    int b = (int)(__float_as_uint(d) - sh.lut_base);
-    b = b < 0 ? 0 : (b >> sh.lut_shift);
-    b = b < sh.nlut - 1 ? b : sh.nlut - 1;
-    int k = lut[b];
-    while (!(d < r2[k])) ++k;  // ends: d < r2[nrads-1] and r2 ascending
-    return k;
+template <int GROUP>
+__device__ __forceinline__ uint32_t grp_scan_max(uint32_t v, int gl) {
+  if constexpr (GROUP == 16) {
+    v = max(v, row_shr<1>(v));
+    v = max(v, row_shr<2>(v));
+    v = max(v, row_shr<4>(v));
+    v = max(v, row_shr<8>(v));
   } else {
+#pragma unroll
+    for (int off = 1; off < GROUP; off <<= 1) {
+      uint32_t t = __shfl_up(v, off, GROUP);
+      if (gl >= off) v = max(v, t);
+    }
+  }
+  return v;
+}
+
+// Value of lane j (group-relative) of this lane's group.
+template <int GROUP>
+__device__ __forceinline__ uint32_t grp_lane(uint32_t v, uint32_t j) {
+  return (uint32_t)__shfl((int)v, (int)j, GROUP);
+}
+
+// Max over the wave of a value that is uniform within each group.
+template <int GROUP>
+__device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
+#pragma unroll
+  for (int off = GROUP; off < 64; off <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+  return v;
+}
+
+// First k with d < r2[k] (src/objects/ArraySTR/MultiRangeCount.c:107-113),
+// as the byte offset 4k of its histogram bin, for a d < r2[nrads-1] (the
+// caller's gate); the scan mode takes any d and returns -1 when no shell
+// matches.
+template <int kShell>
+__device__ __forceinline__ int shell_off(float d, const float* r2, const ShellEntry* lut,
+                                         const Shells& sh) {
+  if constexpr (kShell == kShellScan) {
     for (int k = 0; k < sh.nrads; ++k)
-      if (d < r2[k]) return k;
+      if (d < r2[k]) return 4 * k;
     return -1;
+  } else {
+    // d < r2max bounds the bucket above; bits below the base clamp to 0
+    const int b = max((int)(__float_as_uint(d) - sh.lut_base) >> sh.lut_shift, 0);
+    const ShellEntry e = lut[b];
+    int off = (int)e.off + (d < e.t ? 0 : 4);
+    if constexpr (kShell == kShellTable)
+      while (!(d < r2[off >> 2])) off += 4;  // ends: d < r2[nrads-1], r2 ascending
+    return off;
   }
 }
 
@@ -65,32 +123,60 @@ __device__ __forceinline__ float dpps_distance(float px, float py, float pz, flo
   return sxy + zz;
 }
 
-template <bool kSorted, bool kLds, bool kInbox>
-__global__ __launch_bounds__(kCountThreads) void count_plain_kernel(
+// gfx9 buffer descriptor word 3 (raw buffer, 32-bit data); out-of-range
+// loads return 0, so lanes past the end need no branch.
+constexpr int kBufferFlags = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float* p, uint64_t n) {
+  const uint64_t bytes = n * 4;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0,
+                                           (int)(bytes < 0xffffffffull ? bytes : 0xffffffffull),
+                                           kBufferFlags);
+}
+
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+
+template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox>
+__global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     IndexView ix, HaloView hv, Shells sh, uint64_t* __restrict__ counts,
     uint32_t halos_per_block, int accumulate, unsigned long long* __restrict__ inbox_total) {
+  constexpr int H = 64 / GROUP;
+  // Sorted shells: thresholds are finite or +inf, so a live halo's boxes are
+  // NaN-free and a counted pair needs dist < r2max; each row's x run is cut
+  // to the sphere's chord (with margins), which drops no counted pair.  The
+  // in-box measurement and the any-threshold scan walk the whole box.
+  constexpr bool kSphere = kShell != kShellScan && !kInbox;
   extern __shared__ __align__(16) uint32_t smem[];
   const int K = sh.nrads;
   const int kpad = (K + 3) & ~3;
-  float* s_r2 = reinterpret_cast<float*>(smem);
-  uint16_t* s_lut = reinterpret_cast<uint16_t*>(smem + kpad);
-  uint32_t* s_rows = kLds ? smem + kpad + kLutSize / 2 : smem;
-  uint32_t* s_bins = s_rows + kCountWaves * 128;
+  // LDS: [lut 2*kLutSize][r2 kpad][marks 4 waves x 128][bins 4 waves x H x K]
+  ShellEntry* s_lut = reinterpret_cast<ShellEntry*>(smem);
+  float* s_r2 = reinterpret_cast<float*>(smem + 2 * kLutSize);
+  uint32_t* s_mark = kLds ? smem + 2 * kLutSize + kpad : smem;
+  uint32_t* s_bins = s_mark + kCountWaves * 128;
 
   if (kLds) {
     for (int i = threadIdx.x; i < K; i += kCountThreads) s_r2[i] = sh.r2[i];
-    if (kSorted)
+    if (kShell != kShellScan)
       for (int i = threadIdx.x; i < sh.nlut; i += kCountThreads) s_lut[i] = sh.lut[i];
   }
   __syncthreads();
   const float* r2 = kLds ? s_r2 : sh.r2;
-  const uint16_t* lut = kLds ? s_lut : sh.lut;
+  const ShellEntry* lut = kLds ? s_lut : sh.lut;
 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  uint32_t* rowx = s_rows + wave * 128;
-  uint32_t* rowb = rowx + 64;
-  uint32_t* bins = s_bins + wave * K;
+  const int grp = lane / GROUP;
+  const int gl = lane % GROUP;
+  uint32_t* mark = s_mark + wave * 128 + grp * 2 * GROUP;
+  uint32_t* bins = s_bins + (wave * H + grp) * K;
+  char* bins_bytes = reinterpret_cast<char*>(bins);
+
+  const __amdgpu_buffer_rsrc_t rx = make_rsrc(ix.x, ix.n);
+  const __amdgpu_buffer_rsrc_t ry = make_rsrc(ix.y, ix.n);
+  const __amdgpu_buffer_rsrc_t rz = make_rsrc(ix.z, ix.n);
 
   // Workgroups go round-robin over the 8 XCDs; give each XCD one contiguous
   // range of the (Morton-sorted) halos so its L2 sees neighbouring queries.
@@ -102,26 +188,34 @@ __global__ __launch_bounds__(kCountThreads) void count_plain_kernel(
 
   const float rbox = sh.rbox;
   const float inv = ix.grid.inv_cell;
+  const float cs = ix.grid.cell;
   const int G = ix.grid.g;
   const int gm1 = G - 1;
   uint64_t inbox = 0;
 
-  for (uint64_t hs = h0 + wave; hs < h1; hs += kCountWaves) {
-    const float cx = hv.x[hs], cy = hv.y[hs], cz = hv.z[hs];
-    const uint32_t row = hv.orig[hs];
-    if (kLds) {
-      for (int k = lane; k < K; k += 64) bins[k] = 0u;
-      wave_sync();
+  for (uint64_t hb = h0 + (uint64_t)wave * H; hb < h1; hb += (uint64_t)kCountWaves * H) {
+    const uint64_t hs = hb + grp;
+    const bool active = hs < h1;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    uint32_t row = 0;
+    if (active) {
+      cx = hv.x[hs];
+      cy = hv.y[hs];
+      cz = hv.z[hs];
+      row = hv.orig[hs];
+    }
+    if (kLds && !kInbox) {
+      for (int k = gl; k < K; k += GROUP) bins[k] = 0u;
     }
     // A NaN centre makes every distance NaN: nothing is ever counted.
-    const bool live = (cx == cx) && (cy == cy) && (cz == cz);
-    if (live) {
-      // Periodic images (src/objects/ArraySTR/MultiRangeCount.c:31-71,
-      // src/objects/RTree/Mbr.c:284-313): an axis whose base box leaves
-      // [0, 1000] is shifted by +1000 (low side wins) or -1000.
-      float shift[3] = {0.0f, 0.0f, 0.0f};
+    const bool live = active && (cx == cx) && (cy == cy) && (cz == cz);
+    // Periodic images (src/objects/ArraySTR/MultiRangeCount.c:31-71,
+    // src/objects/RTree/Mbr.c:284-313): an axis whose base box leaves
+    // [0, 1000] is shifted by +1000 (low side wins) or -1000.
+    float shift[3] = {0.0f, 0.0f, 0.0f};
+    int oob = 0;
+    {
       const float c[3] = {cx, cy, cz};
-      int oob = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float lo = c[d] - rbox, hi = c[d] + rbox;
@@ -133,62 +227,130 @@ __global__ __launch_bounds__(kCountThreads) void count_plain_kernel(
           oob |= 1 << d;
         }
       }
-      for (int img = 0; img < 8; ++img) {
-        if (img & ~oob) continue;
-        const float sx = (img & 1) ? cx + shift[0] : cx;
-        const float sy = (img & 2) ? cy + shift[1] : cy;
-        const float sz = (img & 4) ? cz + shift[2] : cz;
-        const float lx = sx - rbox, hx = sx + rbox;
-        const float ly = sy - rbox, hy = sy + rbox;
-        const float lz = sz - rbox, hz = sz + rbox;
-        const int x0 = cell_lo(lx, inv, gm1), x1 = cell_hi(hx, inv, gm1);
-        const int y0 = cell_lo(ly, inv, gm1), y1 = cell_hi(hy, inv, gm1);
-        const int z0 = cell_lo(lz, inv, gm1), z1 = cell_hi(hz, inv, gm1);
-        const int ny = y1 - y0 + 1;
-        const int nrows = ny * (z1 - z0 + 1);
-        for (int rb = 0; rb < nrows; rb += 64) {
-          const int r = rb + lane;
-          uint32_t beg = 0, len = 0;
-          if (r < nrows) {
-            const uint32_t yy = (uint32_t)(y0 + r % ny), zz = (uint32_t)(z0 + r / ny);
-            const uint32_t kb = (zz * (uint32_t)G + yy) * (uint32_t)G;
-            const uint32_t k0 = kb + (uint32_t)x0, k1 = kb + (uint32_t)x1;
+    }
+    // images any halo of this wavefront has (scalar: skipping costs nothing)
+    const uint32_t any_oob = (__ballot(live && (oob & 1)) ? 1u : 0u) |
+                             (__ballot(live && (oob & 2)) ? 2u : 0u) |
+                             (__ballot(live && (oob & 4)) ? 4u : 0u);
+    for (uint32_t img = 0; img < 8; ++img) {
+      if (img & ~any_oob) continue;
+      const bool on = live && !(img & ~(uint32_t)oob);
+      if (!__any(on)) continue;
+      const float sx = (img & 1) ? cx + shift[0] : cx;
+      const float sy = (img & 2) ? cy + shift[1] : cy;
+      const float sz = (img & 4) ? cz + shift[2] : cz;
+      const float lx = sx - rbox, hx = sx + rbox;
+      const float ly = sy - rbox, hy = sy + rbox;
+      const float lz = sz - rbox, hz = sz + rbox;
+      int x0, x1, y0, y1, z0, z1;
+      if constexpr (kSphere) {
+        x0 = cell_of(lx, inv, gm1), x1 = cell_of(hx, inv, gm1);
+        y0 = cell_of(ly, inv, gm1), y1 = cell_of(hy, inv, gm1);
+        z0 = cell_of(lz, inv, gm1), z1 = cell_of(hz, inv, gm1);
+      } else {
+        x0 = cell_lo(lx, inv, gm1), x1 = cell_hi(hx, inv, gm1);
+        y0 = cell_lo(ly, inv, gm1), y1 = cell_hi(hy, inv, gm1);
+        z0 = cell_lo(lz, inv, gm1), z1 = cell_hi(hz, inv, gm1);
+      }
+      const int ny = y1 - y0 + 1;
+      const int nrows = on ? ny * (z1 - z0 + 1) : 0;
+      const int max_rows = (int)wave_max_of_groups<GROUP>((uint32_t)nrows);
+      for (int rb = 0; rb < max_rows; rb += GROUP) {
+        // this lane's row of the batch: start, length, and its segments
+        const int r = rb + gl;
+        uint32_t beg = 0, len = 0;
+        if (r < nrows) {
+          const int zq = r / ny;
+          const int yy = y0 + (r - zq * ny), zz = z0 + zq;
+          int xa = x0, xb = x1;
+          bool keep = true;
+          if constexpr (kSphere) {
+            // the row's (y, z) cell rectangle, widened by 1e-3 for the
+            // binning's rounding (edge cells reach to infinity); a particle
+            // there with dist < r2max has |px - sx| < w
+            const float ylo = yy > 0 ? (float)yy * cs - 1e-3f : -INFINITY;
+            const float yhi = yy < gm1 ? (float)(yy + 1) * cs + 1e-3f : INFINITY;
+            const float zlo = zz > 0 ? (float)zz * cs - 1e-3f : -INFINITY;
+            const float zhi = zz < gm1 ? (float)(zz + 1) * cs + 1e-3f : INFINITY;
+            const float dy = fmaxf(fmaxf(ylo - sy, sy - yhi), 0.0f);
+            const float dz = fmaxf(fmaxf(zlo - sz, sz - zhi), 0.0f);
+            const float rem = sh.r2max_hi - (dy * dy + dz * dz) * 0.99999f;
+            keep = rem > 0.0f;
+            if (keep) {
+              const float w = sqrtf(rem) * 1.00001f + 1e-3f;
+              xa = max(xa, cell_of(sx - w, inv, gm1));
+              xb = min(xb, cell_of(sx + w, inv, gm1));
+            }
+          }
+          if (keep) {
+            const uint32_t kb = ((uint32_t)zz * (uint32_t)G + (uint32_t)yy) * (uint32_t)G;
+            const uint32_t k0 = kb + (uint32_t)xa, k1 = kb + (uint32_t)xb;
             beg = k0 ? ix.cell_end[k0 - 1] : 0u;
             len = ix.cell_end[k1] - beg;
           }
-          const uint32_t incl = wave_incl_sum(len, lane);
-          const uint32_t total = __shfl(incl, 63, 64);
-          if (total == 0) continue;
-          rowx[lane] = incl - len;
-          rowb[lane] = beg;
+        }
+        const uint32_t nseg = (len + SEG - 1) / SEG;
+        const uint32_t sincl = grp_scan_add<GROUP>(nseg, gl);
+        const uint32_t sexcl = sincl - nseg;
+        const uint32_t stotal = grp_lane<GROUP>(sincl, GROUP - 1);
+        const uint32_t max_stotal = wave_max_of_groups<GROUP>(stotal);
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < max_stotal; c0 += GROUP) {
+          // rows whose first segment falls in [c0, c0 + GROUP) mark its
+          // slot; a max-scan over the marks gives every slot its row
+          mark[gl] = 0u;
           wave_sync();
-          const int nr = nrows - rb < 64 ? nrows - rb : 64;
-          for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-            const uint32_t i = c0 + (uint32_t)lane;
-            if (i < total) {
-              int lo = 0, hi = nr;
-              while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (rowx[mid] <= i) lo = mid; else hi = mid;
-              }
-              const uint32_t p = rowb[lo] + (i - rowx[lo]);
-              const float px = ix.x[p], py = ix.y[p], pz = ix.z[p];
-              // closed, NaN-permissive box (src/objects/RTree/Mbr.c:171-182)
-              if (!(px < lx) && !(hx < px) && !(py < ly) && !(hy < py) && !(pz < lz) &&
-                  !(hz < pz)) {
-                if (kInbox) {  // measurement pass: count the pair test only
-                  ++inbox;
-                  continue;
-                }
-                const float dist = dpps_distance(px, py, pz, sx, sy, sz);
-                const int k = shell_of<kSorted>(dist, r2, lut, sh);
-                if (k >= 0) {
+          if (nseg > 0 && sexcl >= c0 && sexcl < c0 + GROUP) mark[sexcl - c0] = (uint32_t)gl + 1;
+          wave_sync();
+          const uint32_t m = grp_scan_max<GROUP>(mark[gl], gl);
+          const uint32_t j = m ? m - 1 : carry;
+          // cross-lane reads run on every lane: one taken inside a branch
+          // would read 0 from lanes the branch left inactive
+          carry = grp_lane<GROUP>(j, GROUP - 1);
+          const uint32_t rbeg = grp_lane<GROUP>(beg, j);
+          const uint32_t rlen = grp_lane<GROUP>(len, j);
+          const uint32_t rsx = grp_lane<GROUP>(sexcl, j);
+          const uint32_t q = c0 + (uint32_t)gl;
+          const uint32_t off = (q - rsx) * SEG;  // segment start within the row
+          const uint32_t slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+          const uint32_t boff = (rbeg + off) * 4u;
+          // a slot past its group's segments loads whatever its offset
+          // names (bounds-checked by the buffer descriptor) and never uses it
+          float px[SEG], py[SEG], pz[SEG];
+#pragma unroll
+          for (int u = 0; u < SEG; ++u) {
+            px[u] = buf_load(rx, boff + 4u * u);
+            py[u] = buf_load(ry, boff + 4u * u);
+            pz[u] = buf_load(rz, boff + 4u * u);
+          }
+#pragma unroll
+          for (int u = 0; u < SEG; ++u) {
+            // closed, NaN-permissive box (src/objects/RTree/Mbr.c:171-182)
+            const bool in = (uint32_t)u < slen && !(px[u] < lx) && !(hx < px[u]) &&
+                            !(py[u] < ly) && !(hy < py[u]) && !(pz[u] < lz) && !(hz < pz[u]);
+            if constexpr (kInbox) {  // measurement pass: count the pair test only
+              inbox += in ? 1 : 0;
+            } else if constexpr (kShell == kShellScan) {
+              if (in) {
+                const int o = shell_off<kShell>(dpps_distance(px[u], py[u], pz[u], sx, sy, sz),
+                                                r2, lut, sh);
+                if (o >= 0) {
                   if (kLds)
-                    atomicAdd(&bins[k], 1u);
+                    atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
                   else
                     atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
-                                  (uint64_t)row * K + k, 1ull);
+                                  (uint64_t)row * K + (o >> 2), 1ull);
                 }
+              }
+            } else {
+              const float dist = dpps_distance(px[u], py[u], pz[u], sx, sy, sz);
+              if (in && dist < sh.r2max) {  // the gate also rejects NaN
+                const int o = shell_off<kShell>(dist, r2, lut, sh);
+                if (kLds)
+                  atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
+                else
+                  atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
+                                (uint64_t)row * K + (o >> 2), 1ull);
               }
             }
           }
@@ -198,10 +360,12 @@ __global__ __launch_bounds__(kCountThreads) void count_plain_kernel(
     }
     if (kLds && !kInbox) {
       wave_sync();
-      uint64_t* dst = counts + (uint64_t)row * K;
-      for (int k = lane; k < K; k += 64) {
-        const uint64_t v = bins[k];
-        if (accumulate) dst[k] += v; else dst[k] = v;
+      if (active) {
+        uint64_t* dst = counts + (uint64_t)row * K;
+        for (int k = gl; k < K; k += GROUP) {
+          const uint64_t v = bins[k];
+          if (accumulate) dst[k] += v; else dst[k] = v;
+        }
       }
       wave_sync();
     }
@@ -214,27 +378,64 @@ __global__ __launch_bounds__(kCountThreads) void count_plain_kernel(
   }
 }
 
-template <bool kSorted, bool kLds, bool kInbox>
-void launch_plain(const IndexView& ix, const HaloView& hv, const Shells& sh,
+template <int GROUP, int kShell, bool kLds, bool kInbox>
+void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh,
                   uint64_t* counts, int accumulate, unsigned long long* inbox,
-                  uint32_t nblocks, size_t lds, hipStream_t s) {
-  count_plain_kernel<kSorted, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
-      ix, hv, sh, counts, kHalosPerBlock, accumulate, inbox);
+                  uint32_t nblocks, size_t lds, int seg, hipStream_t s) {
+  if (seg >= 8)
+    count_group_kernel<GROUP, 8, kShell, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
+        ix, hv, sh, counts, kHalosPerBlock, accumulate, inbox);
+  else
+    count_group_kernel<GROUP, 4, kShell, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
+        ix, hv, sh, counts, kHalosPerBlock, accumulate, inbox);
+}
+
+template <int GROUP, int kShell>
+void dispatch_flags(bool lds, bool in, const IndexView& ix, const HaloView& hv,
+                    const Shells& sh, uint64_t* counts, int acc, unsigned long long* inbox,
+                    uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
+  if (lds && !in) launch_group<GROUP, kShell, true, false>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else if (lds && in) launch_group<GROUP, kShell, true, true>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else if (!in) launch_group<GROUP, kShell, false, false>(ix, hv, sh, counts, 1, inbox, nb, ldsb, seg, s);
+  else launch_group<GROUP, kShell, false, true>(ix, hv, sh, counts, 1, inbox, nb, ldsb, seg, s);
+}
+
+template <int GROUP>
+void dispatch_group(int shell, bool lds, bool in, const IndexView& ix, const HaloView& hv,
+                    const Shells& sh, uint64_t* counts, int acc, unsigned long long* inbox,
+                    uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
+  if (shell == kShellOneStep)
+    dispatch_flags<GROUP, kShellOneStep>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else if (shell == kShellTable)
+    dispatch_flags<GROUP, kShellTable>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else
+    dispatch_flags<GROUP, kShellScan>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
 }
 
 }  // namespace
 
 constexpr size_t kNoLdsBytes = 4 * kCountWaves * 128;
 
-size_t count_lds_bytes(int nrads) {
-  if (nrads > kMaxLdsShells) return 0;
+int count_group_for(int nrads, int requested) {
+  if (requested == 64 || requested == 16) {
+    if ((size_t)nrads * kCountWaves * (64 / requested) * 4 > kMaxGroupBinBytes &&
+        nrads <= kMaxLdsShells)
+      return 64;
+    return requested;
+  }
+  return nrads <= 256 ? 16 : 64;
+}
+
+size_t count_lds_bytes(int nrads, int group) {
+  if (nrads > kMaxLdsShells) return kNoLdsBytes;
   const size_t kpad = (size_t)((nrads + 3) & ~3);
-  return 4 * (kpad + kLutSize / 2 + kCountWaves * 128 + (size_t)kCountWaves * nrads);
+  const size_t h = (size_t)(64 / group);
+  return 4 * (2 * (size_t)kLutSize + kpad + kCountWaves * 128 + kCountWaves * h * nrads);
 }
 
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
                        uint64_t* counts, bool accumulate, unsigned long long* inbox,
-                       hipStream_t s) {
+                       int group, int seg, hipStream_t s) {
   if (hv.n == 0 || sh.nrads <= 0) return hipSuccess;
   const bool lds = sh.nrads <= kMaxLdsShells;
   const bool measure = inbox != nullptr;
@@ -245,22 +446,19 @@ hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh
     if (e != hipSuccess) return e;
   }
   if (ix.n == 0) return hipSuccess;
+  if (ix.n > 0x3fffffffull) return hipErrorInvalidValue;  // 32-bit byte offsets
   uint64_t blocks = ceil_div(hv.n, kHalosPerBlock);
   blocks = (blocks + 7) & ~7ull;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-  const size_t ldsb = count_lds_bytes(sh.nrads);
+  const int g = lds ? count_group_for(sh.nrads, group) : 64;
+  const size_t ldsb = count_lds_bytes(sh.nrads, g);
   const int acc = accumulate ? 1 : 0;
   const uint32_t nb = (uint32_t)blocks;
-  const bool srt = sh.sorted != 0;
-  const bool in = inbox != nullptr;
-  if (srt && lds && !in) launch_plain<true, true, false>(ix, hv, sh, counts, acc, inbox, nb, ldsb, s);
-  else if (srt && lds && in) launch_plain<true, true, true>(ix, hv, sh, counts, acc, inbox, nb, ldsb, s);
-  else if (!srt && lds && !in) launch_plain<false, true, false>(ix, hv, sh, counts, acc, inbox, nb, ldsb, s);
-  else if (!srt && lds && in) launch_plain<false, true, true>(ix, hv, sh, counts, acc, inbox, nb, ldsb, s);
-  else if (srt && !lds && !in) launch_plain<true, false, false>(ix, hv, sh, counts, 1, inbox, nb, kNoLdsBytes, s);
-  else if (srt && !lds && in) launch_plain<true, false, true>(ix, hv, sh, counts, 1, inbox, nb, kNoLdsBytes, s);
-  else if (!srt && !lds && !in) launch_plain<false, false, false>(ix, hv, sh, counts, 1, inbox, nb, kNoLdsBytes, s);
-  else launch_plain<false, false, true>(ix, hv, sh, counts, 1, inbox, nb, kNoLdsBytes, s);
+  const int shell = !sh.sorted ? kShellScan : sh.one_step ? kShellOneStep : kShellTable;
+  if (g == 16)
+    dispatch_group<16>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else
+    dispatch_group<64>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
   return hipGetLastError();
 }
 

@@ -117,6 +117,7 @@ Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli) {
   Grid g;
   g.g = (int)gd;
   g.inv_cell = (float)g.g / 1000.0f;
+  g.cell = 1000.0f / (float)g.g;
   g.ncells = (uint32_t)g.g * (uint32_t)g.g * (uint32_t)g.g;
   return g;
 }
@@ -157,6 +158,8 @@ struct sjcs_engine {
 
   int64_t cell_size_milli = 0;
   int64_t kernel = 0;
+  int64_t lanes_per_halo = 0;
+  int64_t segment = 0;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
@@ -217,7 +220,7 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
   if (!r2 || nrads <= 0) return fail(e, SJCS_ERR_INVALID, "nrads must be >= 1");
   st.nrads = nrads;
   st.r2.assign(r2, r2 + nrads);
-  bool sorted = nrads <= 65535;
+  bool sorted = true;
   for (int k = 0; k < nrads && sorted; ++k) {
     uint32_t b = float_bits(r2[k]);
     if (b >= 0x7f800001u) sorted = false;  // NaN or sign bit set
@@ -227,38 +230,56 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
   v.nrads = nrads;
   v.sorted = sorted ? 1 : 0;
   v.r2max = r2[nrads - 1];
+  v.r2max_hi = r2[nrads - 1] * (1.0f + 1e-5f);
   // src/objects/ArraySTR/MultiRangeCount.c:19: float radius = sqrt(r2[K-1])
   v.rbox = (float)sqrt((double)r2[nrads - 1]);
-  std::vector<uint16_t> lut(1, 0);
+  std::vector<ShellEntry> lut(1, ShellEntry{0u, 0.0f});
   if (sorted) {
-    const uint32_t base = float_bits(r2[0]);
-    const uint32_t range = float_bits(r2[nrads - 1]) - base;
-    int shift = 0;
-    while ((range >> shift) >= (uint32_t)kLutSize) ++shift;
-    const int nlut = (int)(range >> shift) + 1;
-    lut.assign(nlut, 0);
-    for (int b = 0; b < nlut; ++b) {
-      // smallest distance that maps to bucket b (bucket 0 also takes all
-      // distances below r2[0], down to +0)
-      const float dmin = b == 0 ? 0.0f : bits_float(base + ((uint32_t)b << shift));
-      int k = 0;
-      while (k < nrads - 1 && !(dmin < r2[k])) ++k;
-      lut[b] = (uint16_t)k;
+    // prefer a table based at bits 0 (one shift finds the bucket); fall back
+    // to one based at r2[0] when buckets that coarse hold two thresholds
+    const uint32_t top = float_bits(r2[nrads - 1]);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const uint32_t base = attempt == 0 ? 0u : float_bits(r2[0]);
+      const uint32_t range = top - base;
+      int shift = 0;
+      while ((range >> shift) >= (uint32_t)kLutSize) ++shift;
+      const int nlut = (int)(range >> shift) + 1;
+      std::vector<ShellEntry> t(nlut, ShellEntry{0u, 0.0f});
+      bool one_step = true;
+      for (int b = 0; b < nlut; ++b) {
+        // smallest and largest distance of bucket b (bucket 0 also takes
+        // every distance below the base; the last ends below r2max)
+        const float dmin = b == 0 ? 0.0f : bits_float(base + ((uint32_t)b << shift));
+        const uint32_t hib = b + 1 < nlut ? base + ((uint32_t)(b + 1) << shift) - 1u
+                                          : (top > base ? top - 1u : top);
+        const float dmax = bits_float(hib);
+        int k = 0;
+        while (k < nrads - 1 && !(dmin < r2[k])) ++k;
+        t[b].off = 4u * (uint32_t)k;
+        t[b].t = r2[k];
+        // k + 1 is the answer for any d >= r2[k] of this bucket only if the
+        // next threshold lies above the bucket
+        if (k + 1 < nrads && !(dmax < r2[k + 1])) one_step = false;
+      }
+      if (!one_step && attempt == 0) continue;
+      lut.swap(t);
+      v.lut_base = base;
+      v.lut_shift = shift;
+      v.nlut = nlut;
+      v.one_step = one_step ? 1 : 0;
+      break;
     }
-    v.lut_base = base;
-    v.lut_shift = shift;
-    v.nlut = nlut;
   }
   HIPCALL(e, ensure(st.d_r2, sizeof(float) * nrads), "hipMalloc(shells)");
-  HIPCALL(e, ensure(st.d_lut, sizeof(uint16_t) * lut.size()), "hipMalloc(shell table)");
+  HIPCALL(e, ensure(st.d_lut, sizeof(ShellEntry) * lut.size()), "hipMalloc(shell table)");
   HIPCALL(e, hipMemcpyAsync(st.d_r2.p, r2, sizeof(float) * nrads, hipMemcpyHostToDevice, e->stream),
           "hipMemcpy(shells)");
-  HIPCALL(e, hipMemcpyAsync(st.d_lut.p, lut.data(), sizeof(uint16_t) * lut.size(),
+  HIPCALL(e, hipMemcpyAsync(st.d_lut.p, lut.data(), sizeof(ShellEntry) * lut.size(),
                             hipMemcpyHostToDevice, e->stream),
           "hipMemcpy(shell table)");
   HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
   v.r2 = st.d_r2.as<float>();
-  v.lut = st.d_lut.as<uint16_t>();
+  v.lut = st.d_lut.as<ShellEntry>();
   st.view = v;
   return SJCS_OK;
 }
@@ -385,13 +406,19 @@ int prepare_counts(sjcs_engine* e) {
   return SJCS_OK;
 }
 
+// Lanes per halo of the count kernel: "kernel" 1 forces the plain one
+// halo per wavefront kernel, else "lanes_per_halo" (0 = automatic).
+int lanes(const sjcs_engine* e) { return e->kernel == 1 ? 64 : (int)e->lanes_per_halo; }
+int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
+
 int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli);
   int rc = build_index(e, e->ib, d_xyz, n, g);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
   HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view,
-                         e->counts.as<uint64_t>(), !e->counts_fresh, nullptr, e->stream),
+                         e->counts.as<uint64_t>(), !e->counts_fresh, nullptr, lanes(e),
+                         seg_len(e), e->stream),
           "count kernel");
   HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
   e->counts_fresh = false;
@@ -555,7 +582,7 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
   if (!e->timed) return SJCS_OK;
   HIPCALL(e, fill_u64(e->inbox.as<uint64_t>(), 0ull, 1, e->stream), "fill");
   HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view, nullptr, true,
-                         e->inbox.as<unsigned long long>(), e->stream),
+                         e->inbox.as<unsigned long long>(), lanes(e), seg_len(e), e->stream),
           "pair-test count");
   HIPCALL(e, hipMemcpyAsync(pair_tests, e->inbox.p, 8, hipMemcpyDeviceToHost, e->stream),
           "hipMemcpy");
@@ -569,6 +596,18 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     if (value < 0 || (value > 0 && value < 977))  // at most 1024 cells per axis
       return fail(e, SJCS_ERR_INVALID, "cell_size_milli: 0 or >= 977");
     e->cell_size_milli = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "segment") == 0) {
+    if (value != 0 && value != 4 && value != 8)
+      return fail(e, SJCS_ERR_INVALID, "segment: 0, 4 or 8");
+    e->segment = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "lanes_per_halo") == 0) {
+    if (value != 0 && value != 16 && value != 64)
+      return fail(e, SJCS_ERR_INVALID, "lanes_per_halo: 0, 16 or 64");
+    e->lanes_per_halo = value;
     return SJCS_OK;
   }
   if (strcmp(key, "kernel") == 0) {
@@ -693,7 +732,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
     hipError_t h = ensure(dcounts, nc * sizeof(uint64_t));
     if (h == hipSuccess)
       h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint64_t>(), false,
-                      nullptr, e->stream);
+                      nullptr, lanes(e), seg_len(e), e->stream);
     if (h == hipSuccess) {
       host.resize(nc);
       h = hipMemcpyAsync(host.data(), dcounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
