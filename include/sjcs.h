@@ -108,6 +108,8 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *                       automatic (16 up to 256 shells)
  *   "segment"           particles of a cell row one lane walks: 4 or 8,
  *                       0 = automatic
+ *   "index_sort"        0 = automatic (axis by axis up to 256 cells per
+ *                       axis), 1 = always by the packed cell key
  * Returns SJCS_ERR_INVALID for an unknown key or value. */
 int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
 

@@ -97,15 +97,16 @@ int key_bits_for(uint32_t ncells) {
   return b;
 }
 
-// Cell edge: half the query half-width by default (a query box then spans
-// about 5 cells per axis), coarsened until there are at most ~2 cells per
-// particle, and at most 1024 cells per axis.
+// Cell edge: 0.9 of the query half-width by default (a query box then spans
+// about 3 cells per axis, and its sphere cuts most of each row), coarsened
+// until there are at most ~2 cells per particle, and at most 1024 cells per
+// axis.
 Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli) {
   double s;
   if (cell_milli > 0)
     s = (double)cell_milli / 1000.0;
   else if (rbox > 0.0f && isfinite(rbox))
-    s = (double)rbox / 2.0;
+    s = (double)rbox * 0.9;
   else
     s = 1000.0;
   double gd = ceil(1000.0 / s);
@@ -160,6 +161,7 @@ struct sjcs_engine {
   int64_t kernel = 0;
   int64_t lanes_per_halo = 0;
   int64_t segment = 0;
+  int64_t index_sort = 0;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
@@ -359,17 +361,31 @@ int upload_points(sjcs_engine* e, DevBuf& dst, const float* pts, size_t stride, 
 int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles per chunk");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
+  // grids of at most kCsMaxCells cells per axis use the per-axis cell sort;
+  // finer ones (or "index_sort" 1) the radix sort of packed cell keys
+  const bool by_axis = g.g <= kCsMaxCells && e->index_sort != 1;
   for (int i = 0; i < 2; ++i) {
-    HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
+    if (!by_axis) HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.x[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.y[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.z[i], b), "hipMalloc(index)");
   }
   HIPCALL(e, ensure(ib.cell_end, sizeof(uint32_t) * (size_t)g.ncells), "hipMalloc(cell table)");
-  int rc = ensure_ws(e, std::max(radix_workspace_words(n), scan_workspace_words(g.ncells)));
+  int rc = ensure_ws(e, by_axis ? cell_sort_workspace_words(n)
+                                : std::max(radix_workspace_words(n), scan_workspace_words(g.ncells)));
   if (rc != SJCS_OK) return rc;
   ib.grid = g;
   ib.n = n;
+  if (by_axis) {
+    float* xs[2] = {ib.x[0].as<float>(), ib.x[1].as<float>()};
+    float* ys[2] = {ib.y[0].as<float>(), ib.y[1].as<float>()};
+    float* zs[2] = {ib.z[0].as<float>(), ib.z[1].as<float>()};
+    HIPCALL(e, cell_sort_particles(d_xyz, n, g, xs, ys, zs, ib.cell_end.as<uint32_t>(),
+                                   e->ws.as<uint32_t>(), e->stream),
+            "cell sort");
+    ib.sel = 0;
+    return SJCS_OK;
+  }
   HIPCALL(e, particle_keys(d_xyz, n, g, ib.key[0].as<uint32_t>(), ib.x[0].as<float>(),
                            ib.y[0].as<float>(), ib.z[0].as<float>(), e->stream),
           "cell keys");
@@ -608,6 +624,11 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     if (value != 0 && value != 16 && value != 64)
       return fail(e, SJCS_ERR_INVALID, "lanes_per_halo: 0, 16 or 64");
     e->lanes_per_halo = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "index_sort") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "index_sort: 0 or 1");
+    e->index_sort = value;
     return SJCS_OK;
   }
   if (strcmp(key, "kernel") == 0) {

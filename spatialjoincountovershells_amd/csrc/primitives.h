@@ -20,6 +20,16 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+// Cell sort (grids of at most kCsMaxCells cells per axis): tiles of the
+// two scatter passes, and rows of the last pass staged in LDS.
+constexpr int kCsThreads = 512;
+constexpr int kCsItems = 8;
+constexpr int kCsTile = kCsThreads * kCsItems;  // 4096 particles per block
+constexpr int kCsRowThreads = 256;
+constexpr int kCsRowItems = 16;
+constexpr int kCsRowCap = kCsRowThreads * kCsRowItems;  // 4096 particles
+constexpr int kCsMaxCells = 256;
+
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Workspace (u32 words) a radix sort of n keys needs: per-tile digit
@@ -28,6 +38,14 @@ inline uint64_t radix_workspace_words(uint64_t n) {
   uint64_t tiles = ceil_div(n ? n : 1, kSortTile);
   uint64_t hist = (uint64_t)kRadix * tiles;
   return hist + ceil_div(hist, kScanTile) + 64;
+}
+
+// Workspace (u32 words) of cell_sort_particles for n particles.
+inline uint64_t cell_sort_workspace_words(uint64_t n) {
+  uint64_t tiles1 = ceil_div(n ? n : 1, kCsTile);
+  uint64_t tiles2 = tiles1 + kCsMaxCells;
+  uint64_t hist = (uint64_t)kCsMaxCells * (tiles1 + tiles2);
+  return hist + 2 * (kCsMaxCells + 1) + ceil_div((uint64_t)kCsMaxCells * tiles2, kScanTile) + 64;
 }
 
 // Workspace (u32 words) for a scan of n words.
@@ -51,6 +69,13 @@ hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
 hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay,
                             uint64_t n, int key_bits, uint32_t* ws,
                             hipStream_t s, int* result);
+
+// Particle chunk (packed xyz, n < 2^32) -> SoA coordinates in cell-key
+// order in x[0], y[0], z[0] (x[1], y[1], z[1] are scratch), and the cell
+// table: cell_end[c] = number of particles in cells <= c.  Needs g.g <=
+// kCsMaxCells.  The order of the particles inside one cell is unspecified.
+hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
+                               float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s);
 
 // Particle chunk (packed xyz) -> cell keys + SoA coordinates.
 hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key,

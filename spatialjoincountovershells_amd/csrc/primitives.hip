@@ -311,6 +311,303 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
   }
 }
 
+// ------------------------------------------------------------- cell sort ----
+//
+// For grids of at most kCsMaxCells cells per axis, particles are put in cell
+// order (cz, cy, cx) by three MSD passes whose digits are single coordinates'
+// cell indices, recomputed from the coordinate each time (no key array):
+//   1. by cz over the whole chunk, packed xyz in, SoA out: per-tile digit
+//      histograms, one exclusive scan over them (digit-major), a scatter;
+//   2. by cy inside every cz slab: tiles never straddle a slab and the
+//      histograms are laid out [slab][cy][tile of the slab], so again one
+//      exclusive scan gives every (tile, digit) its output offset;
+//   3. by cx inside every (cz, cy) row: one block per row counts, scans,
+//      writes the row's entries of the cell table and writes the row out in
+//      cx order.
+// Ranks inside a tile come from LDS atomics, so the order of the particles
+// of one cell is not kept; nothing downstream depends on it, every count
+// being a sum over whole cells.
+
+// Blocks go round-robin over the 8 XCDs; with a grid of 8 * ceil(ntiles / 8)
+// blocks this gives each XCD one contiguous range of tiles, so the partial
+// cache lines where one tile's digit run meets the next tile's are written
+// through the same L2.  Returns a value >= ntiles for a block with no tile.
+__device__ __forceinline__ uint32_t xcd_tile() {
+  const uint32_t per = gridDim.x >> 3;
+  return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+// Items of a tile, and where its digit-d count lives: hist[hbase + d * hstride].
+struct TileRange {
+  uint64_t begin;
+  uint32_t len;
+  uint64_t hbase;
+  uint32_t hstride;
+};
+
+__device__ __forceinline__ bool tile_range_chunk(uint32_t t, uint32_t ntiles, uint64_t n,
+                                                 TileRange& r) {
+  if (t >= ntiles) return false;
+  r.begin = (uint64_t)t * kCsTile;
+  const uint64_t left = n - r.begin;
+  r.len = left < (uint64_t)kCsTile ? (uint32_t)left : (uint32_t)kCsTile;
+  r.hbase = t;
+  r.hstride = ntiles;
+  return true;
+}
+
+// Tile t of the slab-major tiling: slab s starts at ss[s] and owns tiles
+// [tb[s], tb[s+1]); tb[G] is the number of tiles.
+__device__ __forceinline__ bool tile_range_slab(uint32_t t, const uint32_t* ss,
+                                                const uint32_t* tb, int G, TileRange& r) {
+  if (t >= tb[G]) return false;
+  int lo = 0, hi = G - 1;  // the last slab s with tb[s] <= t; then t < tb[s + 1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tb[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const uint32_t i = t - tb[lo];
+  const uint32_t ts = tb[lo + 1] - tb[lo];
+  r.begin = (uint64_t)ss[lo] + (uint64_t)i * kCsTile;
+  const uint64_t left = (uint64_t)ss[lo + 1] - r.begin;
+  r.len = left < (uint64_t)kCsTile ? (uint32_t)left : (uint32_t)kCsTile;
+  r.hbase = (uint64_t)tb[lo] * (uint64_t)G + i;
+  r.hstride = ts;
+  return true;
+}
+
+// Per-tile histogram of the cells of coordinate c[i * stride].
+template <bool SLAB>
+__global__ __launch_bounds__(kCsThreads) void cs_hist_kernel(
+    const float* __restrict__ c, uint32_t stride, uint64_t n, float inv, int G, uint32_t ntiles,
+    const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kCsMaxCells];
+  TileRange r;
+  const uint32_t t = xcd_tile();
+  if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kCsMaxCells; i += kCsThreads) s_h[i] = 0;
+  __syncthreads();
+  const float* cb = c + r.begin * stride;
+  float v[kCsItems];
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j) {
+    const uint32_t i = (uint32_t)(j * kCsThreads + tid);
+    v[j] = i < r.len ? cb[(uint64_t)i * stride] : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j)
+    if ((uint32_t)(j * kCsThreads + tid) < r.len) atomicAdd(&s_h[cell_of(v[j], inv, G - 1)], 1u);
+  __syncthreads();
+  for (int d = tid; d < G; d += kCsThreads) hist[r.hbase + (uint64_t)d * r.hstride] = s_h[d];
+}
+
+struct Packed3 {
+  float x, y, z;
+};
+
+// Scatter of one tile by the cell of coordinate AXIS to the offsets of the
+// scanned histograms.  Input packed xyz (AOS) or SoA; output SoA.
+template <int AXIS, bool AOS, bool SLAB>
+__global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
+    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, uint64_t n, Grid g,
+    uint32_t ntiles, const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb,
+    const uint32_t* __restrict__ hist) {
+  __shared__ float s_c[3][kCsTile];
+  __shared__ uint32_t s_cnt[kCsMaxCells];
+  __shared__ uint32_t s_off[kCsMaxCells];
+  __shared__ uint32_t s_tmp[kCsThreads / 64];
+  const int G = g.g, gm1 = G - 1;
+  const float inv = g.inv_cell;
+  TileRange r;
+  const uint32_t t = xcd_tile();
+  if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kCsMaxCells; i += kCsThreads) s_cnt[i] = 0;
+
+  float px[kCsItems], py[kCsItems], pz[kCsItems];
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j) {
+    const uint32_t i = (uint32_t)(j * kCsThreads + tid);
+    px[j] = py[j] = pz[j] = 0.0f;
+    if (i < r.len) {
+      if (AOS) {
+        const Packed3 p = reinterpret_cast<const Packed3*>(ix)[r.begin + i];
+        px[j] = p.x;
+        py[j] = p.y;
+        pz[j] = p.z;
+      } else {
+        px[j] = ix[r.begin + i];
+        py[j] = iy[r.begin + i];
+        pz[j] = iz[r.begin + i];
+      }
+    }
+  }
+  __syncthreads();
+
+  uint32_t rk[kCsItems];  // digit << 16 | rank among the tile's items of that digit
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j) {
+    rk[j] = 0;
+    if ((uint32_t)(j * kCsThreads + tid) < r.len) {
+      const float v = AXIS == 0 ? px[j] : (AXIS == 1 ? py[j] : pz[j]);
+      const uint32_t d = (uint32_t)cell_of(v, inv, gm1);
+      rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
+    }
+  }
+  __syncthreads();
+  const uint32_t c = tid < kCsMaxCells ? s_cnt[tid] : 0u;
+  uint32_t total;
+  const uint32_t ls = block_excl_scan<OpSum, kCsThreads>(c, s_tmp, &total);
+  if (tid < G) {
+    // out index = offset of the tile's digit run + (slot - local start);
+    // u32 arithmetic wraps back into range
+    s_off[tid] = hist[r.hbase + (uint64_t)tid * r.hstride] - ls;
+    s_cnt[tid] = ls;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j) {
+    if ((uint32_t)(j * kCsThreads + tid) < r.len) {
+      const uint32_t lp = s_cnt[rk[j] >> 16] + (rk[j] & 0xffffu);
+      s_c[0][lp] = px[j];
+      s_c[1][lp] = py[j];
+      s_c[2][lp] = pz[j];
+    }
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < r.len; l += kCsThreads) {
+    const float x = s_c[0][l], y = s_c[1][l], z = s_c[2][l];
+    const float v = AXIS == 0 ? x : (AXIS == 1 ? y : z);
+    const uint32_t gp = s_off[cell_of(v, inv, gm1)] + l;
+    ox[gp] = x;
+    oy[gp] = y;
+    oz[gp] = z;
+  }
+}
+
+// Slab starts and the slab-major tiling of pass 2, from pass 1's scanned
+// histograms.  One block of kCsMaxCells threads.
+__global__ __launch_bounds__(kCsMaxCells) void cs_slab_tiles_kernel(
+    const uint32_t* __restrict__ hist1, uint32_t tiles1, uint64_t n, int G,
+    uint32_t* __restrict__ ss, uint32_t* __restrict__ tb) {
+  __shared__ uint32_t s_tmp[kCsMaxCells / 64];
+  const int tid = threadIdx.x;
+  const uint32_t st = tid < G ? hist1[(uint64_t)tid * tiles1] : 0u;
+  const uint32_t en = tid + 1 < G ? hist1[(uint64_t)(tid + 1) * tiles1] : (uint32_t)n;
+  const uint32_t nt = tid < G ? (uint32_t)(((uint64_t)(en - st) + kCsTile - 1) / kCsTile) : 0u;
+  uint32_t total;
+  const uint32_t excl = block_excl_scan<OpSum, kCsMaxCells>(nt, s_tmp, &total);
+  if (tid < G) {
+    ss[tid] = st;
+    tb[tid] = excl;
+  }
+  if (tid == 0) {
+    ss[G] = (uint32_t)n;
+    tb[G] = total;
+  }
+}
+
+// First particle of row rw = cz * G + cy after pass 2 (rw = G*G: n).
+__device__ __forceinline__ uint32_t row_begin(uint32_t rw, int G, uint64_t n,
+                                              const uint32_t* ss, const uint32_t* tb,
+                                              const uint32_t* hist2) {
+  if (rw >= (uint32_t)(G * G)) return (uint32_t)n;
+  const uint32_t sl = rw / (uint32_t)G, d = rw - sl * (uint32_t)G;
+  const uint32_t ts = tb[sl + 1] - tb[sl];
+  return ts ? hist2[(uint64_t)tb[sl] * (uint64_t)G + (uint64_t)d * ts] : ss[sl];
+}
+
+// Pass 3: every row in cx order, and the cell table.  Rows of up to
+// kCsRowCap particles are staged in LDS and written out contiguously;
+// longer ones are counted and then scattered straight from HBM.
+__global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
+    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+    uint32_t* __restrict__ cell_end, uint64_t n, Grid g, const uint32_t* __restrict__ ss,
+    const uint32_t* __restrict__ tb, const uint32_t* __restrict__ hist2, uint32_t nrows) {
+  __shared__ float s_c[3][kCsRowCap];
+  __shared__ uint32_t s_cnt[kCsMaxCells];
+  __shared__ uint32_t s_tmp[kCsRowThreads / 64];
+  const uint32_t rw = xcd_tile();
+  if (rw >= nrows) return;
+  const int G = g.g, gm1 = G - 1;
+  const float inv = g.inv_cell;
+  const int tid = threadIdx.x;
+  const uint32_t start = row_begin(rw, G, n, ss, tb, hist2);
+  const uint32_t len = row_begin(rw + 1, G, n, ss, tb, hist2) - start;
+  for (int i = tid; i < kCsMaxCells; i += kCsRowThreads) s_cnt[i] = 0;
+  __syncthreads();
+  uint32_t* ce = cell_end + (uint64_t)rw * (uint64_t)G;
+
+  if (len <= (uint32_t)kCsRowCap) {
+    float px[kCsRowItems], py[kCsRowItems], pz[kCsRowItems];
+    uint32_t rk[kCsRowItems];
+#pragma unroll
+    for (int j = 0; j < kCsRowItems; ++j) {
+      const uint32_t i = (uint32_t)(j * kCsRowThreads + tid);
+      px[j] = py[j] = pz[j] = 0.0f;
+      if (i < len) {
+        px[j] = ix[start + i];
+        py[j] = iy[start + i];
+        pz[j] = iz[start + i];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kCsRowItems; ++j) {
+      rk[j] = 0;
+      if ((uint32_t)(j * kCsRowThreads + tid) < len) {
+        const uint32_t d = (uint32_t)cell_of(px[j], inv, gm1);
+        rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
+      }
+    }
+    __syncthreads();
+    const uint32_t c = s_cnt[tid];
+    uint32_t total;
+    const uint32_t ls = block_excl_scan<OpSum, kCsRowThreads>(c, s_tmp, &total);
+    if (tid < G) {
+      ce[tid] = start + ls + c;
+      s_cnt[tid] = ls;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kCsRowItems; ++j) {
+      if ((uint32_t)(j * kCsRowThreads + tid) < len) {
+        const uint32_t lp = s_cnt[rk[j] >> 16] + (rk[j] & 0xffffu);
+        s_c[0][lp] = px[j];
+        s_c[1][lp] = py[j];
+        s_c[2][lp] = pz[j];
+      }
+    }
+    __syncthreads();
+    for (uint32_t l = tid; l < len; l += kCsRowThreads) {
+      ox[start + l] = s_c[0][l];
+      oy[start + l] = s_c[1][l];
+      oz[start + l] = s_c[2][l];
+    }
+  } else {
+    for (uint32_t l = tid; l < len; l += kCsRowThreads)
+      atomicAdd(&s_cnt[cell_of(ix[start + l], inv, gm1)], 1u);
+    __syncthreads();
+    const uint32_t c = s_cnt[tid];
+    uint32_t total;
+    const uint32_t ls = block_excl_scan<OpSum, kCsRowThreads>(c, s_tmp, &total);
+    if (tid < G) {
+      ce[tid] = start + ls + c;
+      s_cnt[tid] = start + ls;  // running output cursor of every cell
+    }
+    __syncthreads();
+    for (uint32_t l = tid; l < len; l += kCsRowThreads) {
+      const float x = ix[start + l], y = iy[start + l], z = iz[start + l];
+      const uint32_t p = atomicAdd(&s_cnt[cell_of(x, inv, gm1)], 1u);
+      ox[p] = x;
+      oy[p] = y;
+      oz[p] = z;
+    }
+  }
+}
+
 // -------------------------------------------------------------- key make ----
 
 __global__ void particle_keys_kernel(const float* __restrict__ xyz, uint64_t n, Grid g,
@@ -426,6 +723,45 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay, ui
   }
   *result = cur;
   return hipSuccess;
+}
+
+hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
+                               float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s) {
+  const int G = g.g;
+  if (G > kCsMaxCells || G < 1) return hipErrorInvalidValue;
+  if (n > 0xffffffffull) return hipErrorInvalidValue;
+  if (n == 0) return fill_u32(cell_end, 0u, g.ncells, s);
+  const uint64_t tiles1 = ceil_div(n, kCsTile);
+  const uint64_t tiles2 = tiles1 + (uint64_t)G;  // each slab adds at most one partial tile
+  if (tiles2 * (uint64_t)G > 0xffffffffull) return hipErrorInvalidValue;
+  uint32_t* hist1 = ws;
+  uint32_t* hist2 = hist1 + (uint64_t)kCsMaxCells * tiles1;
+  uint32_t* ss = hist2 + (uint64_t)kCsMaxCells * tiles2;
+  uint32_t* tb = ss + kCsMaxCells + 1;
+  uint32_t* scan_ws = tb + kCsMaxCells + 1;
+  const uint32_t nt1 = (uint32_t)tiles1;
+  const uint32_t nb1 = (uint32_t)(8 * ceil_div(tiles1, 8));
+  const uint32_t nb2 = (uint32_t)(8 * ceil_div(tiles2, 8));
+  const uint32_t nrows = (uint32_t)(G * G);
+  const uint32_t nb3 = (uint32_t)(8 * ceil_div(nrows, 8));
+  // pass 1: by cz, packed xyz -> buffers 0
+  cs_hist_kernel<false><<<nb1, kCsThreads, 0, s>>>(xyz + 2, 3u, n, g.inv_cell, G, nt1, nullptr,
+                                                    nullptr, hist1);
+  hipError_t e = exclusive_sum_u32(hist1, hist1, (uint64_t)G * tiles1, scan_ws, s);
+  if (e != hipSuccess) return e;
+  cs_scatter_kernel<2, true, false><<<nb1, kCsThreads, 0, s>>>(
+      xyz, nullptr, nullptr, x[0], y[0], z[0], n, g, nt1, nullptr, nullptr, hist1);
+  cs_slab_tiles_kernel<<<1, kCsMaxCells, 0, s>>>(hist1, nt1, n, G, ss, tb);
+  // pass 2: by cy inside each slab, buffers 0 -> 1
+  cs_hist_kernel<true><<<nb2, kCsThreads, 0, s>>>(y[0], 1u, n, g.inv_cell, G, 0u, ss, tb, hist2);
+  e = exclusive_sum_u32(hist2, hist2, (uint64_t)G * tiles2, scan_ws, s);
+  if (e != hipSuccess) return e;
+  cs_scatter_kernel<1, false, true><<<nb2, kCsThreads, 0, s>>>(
+      x[0], y[0], z[0], x[1], y[1], z[1], n, g, 0u, ss, tb, hist2);
+  // pass 3: by cx inside each row, buffers 1 -> 0, and the cell table
+  cs_row_kernel<<<nb3, kCsRowThreads, 0, s>>>(x[1], y[1], z[1], x[0], y[0], z[0], cell_end, n, g,
+                                              ss, tb, hist2, nrows);
+  return hipGetLastError();
 }
 
 hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key, float* x,

@@ -185,16 +185,43 @@ def test_special_coordinates(eng):
     assert got[0].sum() == 0 and got[8].sum() == 0
 
 
-@pytest.mark.parametrize("cell_milli", [977, 1500, 2500, 4999, 10000, 50000, 250000, 1000000])
-def test_cell_size_does_not_change_counts(eng, cell_milli):
+@pytest.mark.parametrize("index_sort", [0, 1])
+@pytest.mark.parametrize("cell_milli", [977, 1500, 2500, 3907, 4999, 10000, 50000, 250000,
+                                        1000000])
+def test_cell_size_does_not_change_counts(eng, cell_milli, index_sort):
+    """Every cell size, with the per-axis sort (up to 256 cells per axis)
+    and with the packed-key sort."""
     p, h = gen_uniform(15, 30_000, 2_000)
     _, r2 = t.oracle_thresholds(0.001, 5.0, 8)
     want, _ = oracle_chunks([p], h, r2)
     eng.set_option("cell_size_milli", cell_milli)
+    eng.set_option("index_sort", index_sort)
     try:
         got = run_engine(eng, [p], h, r2)
     finally:
         eng.set_option("cell_size_milli", 0)
+        eng.set_option("index_sort", 0)
+    assert_counts_equal(got, want, h)
+
+
+@pytest.mark.parametrize("index_sort", [0, 1])
+def test_clustered_particles(eng, index_sort):
+    """Most particles in one tight clump (whole sort tiles of one cell, one
+    digit per wavefront), halos around and inside it, plus a uniform
+    background; chunk sizes off the tile size."""
+    rng = np.random.default_rng(19)
+    clump = (np.float32(500.0) + rng.normal(0, 0.4, (70_001, 3))).astype(np.float32)
+    bg = rng.random((9_999, 3), dtype=np.float32) * np.float32(1000.0)
+    p = np.concatenate([clump[:40_000], bg, clump[40_000:]])
+    h = (np.float32(500.0) + rng.uniform(-6, 6, (300, 3))).astype(np.float32)
+    h = np.concatenate([h, rng.random((200, 3), dtype=np.float32) * np.float32(1000.0)])
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 8)
+    eng.set_option("index_sort", index_sort)
+    try:
+        got = run_engine(eng, [p[:16_385], p[16_385:]], h, r2)
+    finally:
+        eng.set_option("index_sort", 0)
+    want, _ = oracle_chunks([p[:16_385], p[16_385:]], h, r2)
     assert_counts_equal(got, want, h)
 
 
@@ -264,6 +291,11 @@ def test_full_size_config2_properties(eng):
         eng.reset_counts()
         eng.add_particles_device(d, p.shape[0])
         assert np.array_equal(eng.counts(), base)
+        eng.set_option("index_sort", 1)
+        eng.reset_counts()
+        eng.add_particles_device(d, p.shape[0])
+        assert np.array_equal(eng.counts(), base)
+        eng.set_option("index_sort", 0)
         eng.set_option("cell_size_milli", 0)
         eng.reset_counts()
         third = p.shape[0] // 3
@@ -272,6 +304,7 @@ def test_full_size_config2_properties(eng):
         assert np.array_equal(eng.counts(), base)
     finally:
         eng.set_option("cell_size_milli", 0)
+        eng.set_option("index_sort", 0)
         eng.device_free(d)
     sample = rng.choice(h.shape[0], 12, replace=False)
     want, _ = t.oracle_count(p, h[sample], r2, threads=16)
