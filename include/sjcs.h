@@ -104,7 +104,7 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
 /* Tuning knobs (no reference counterpart).  Keys:
  *   "cell_size_milli"   cell edge in 1/1000 box units, 0 = automatic
  *   "kernel"            0 = automatic, 1 = one halo per wavefront (plain)
- *   "lanes_per_halo"    16 or 64 lanes of a wavefront per halo, 0 =
+ *   "lanes_per_halo"    8, 16 or 64 lanes of a wavefront per halo, 0 =
  *                       automatic (16 up to 256 shells)
  *   "segment"           particles of a cell row one lane walks: 4 or 8,
  *                       0 = automatic

@@ -17,7 +17,7 @@ constexpr uint32_t kHalosPerBlock = 64;
 constexpr size_t kMaxGroupBinBytes = 32 * 1024;
 
 // Lanes per halo the count kernel uses for nrads shells when `requested`
-// lanes (16 or 64; anything else = automatic) are asked for.
+// lanes (8, 16 or 64; anything else = automatic) are asked for.
 int count_group_for(int nrads, int requested);
 
 // Dynamic LDS of the count kernel for nrads shells at `group` lanes/halo.

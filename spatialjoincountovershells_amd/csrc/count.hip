@@ -5,7 +5,8 @@
 // threadJoin (src/astr_rcount/main.c:795-826).
 //
 // One halo per GROUP lanes of a wavefront (GROUP 64: the plain
-// one-halo-per-wavefront kernel; 16: four halos in flight per wavefront).
+// one-halo-per-wavefront kernel; 16 or 8: four or eight halos in flight per
+// wavefront).
 // For the halo and each of its periodic images the query box's cell range is
 // cut into (y, z) rows, whose x runs are contiguous in the sorted particle
 // arrays.  Row r of a batch belongs to lane r of the group.  Rows are cut
@@ -39,7 +40,9 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xf, 0xf, false);
 }
 
-// Inclusive sum / max over the GROUP lanes of a group (values >= 0).
+// Inclusive sum / max over the GROUP lanes of a group (values >= 0).  For
+// GROUP 8 a DPP row holds two groups, so a step that would reach into the
+// group below is masked.
 template <int GROUP>
 __device__ __forceinline__ uint32_t grp_scan_add(uint32_t v, int gl) {
   if constexpr (GROUP == 16) {
@@ -47,6 +50,13 @@ __device__ __forceinline__ uint32_t grp_scan_add(uint32_t v, int gl) {
     v += row_shr<2>(v);
     v += row_shr<4>(v);
     v += row_shr<8>(v);
+  } else if constexpr (GROUP == 8) {
+    uint32_t t = row_shr<1>(v);
+    v += gl >= 1 ? t : 0u;
+    t = row_shr<2>(v);
+    v += gl >= 2 ? t : 0u;
+    t = row_shr<4>(v);
+    v += gl >= 4 ? t : 0u;
   } else {
 #pragma unroll
     for (int off = 1; off < GROUP; off <<= 1) {
@@ -64,6 +74,13 @@ __device__ __forceinline__ uint32_t grp_scan_max(uint32_t v, int gl) {
     v = max(v, row_shr<2>(v));
     v = max(v, row_shr<4>(v));
     v = max(v, row_shr<8>(v));
+  } else if constexpr (GROUP == 8) {
+    uint32_t t = row_shr<1>(v);
+    v = max(v, gl >= 1 ? t : 0u);
+    t = row_shr<2>(v);
+    v = max(v, gl >= 2 ? t : 0u);
+    t = row_shr<4>(v);
+    v = max(v, gl >= 4 ? t : 0u);
   } else {
 #pragma unroll
     for (int off = 1; off < GROUP; off <<= 1) {
@@ -242,6 +259,17 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       const float lx = sx - rbox, hx = sx + rbox;
       const float ly = sy - rbox, hy = sy + rbox;
       const float lz = sz - rbox, hz = sz + rbox;
+      // Below `safe` a distance proves the box test: with m = max(|sx|, |sy|,
+      // |sz|) and A = rbox - (m + rbox) * 2^-21, dist < A^2 (1 - 2^-19) gives
+      // |p - s| < A (1 + 2^-24) on every axis (each squared term is at most
+      // the FP32 sum), which keeps p inside [fl(s - rbox), fl(s + rbox)] with
+      // room for those two roundings.  A non-finite centre gets no bound.
+      float safe = 0.0f;
+      if constexpr (kSphere) {
+        const float m = fmaxf(fmaxf(fabsf(sx), fabsf(sy)), fabsf(sz));
+        const float a = rbox - (m + rbox) * 0x1p-21f;
+        safe = a > 0.0f ? (a * a) * (1.0f - 0x1p-19f) : 0.0f;
+      }
       int x0, x1, y0, y1, z0, z1;
       if constexpr (kSphere) {
         x0 = cell_of(lx, inv, gm1), x1 = cell_of(hx, inv, gm1);
@@ -260,7 +288,9 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         const int r = rb + gl;
         uint32_t beg = 0, len = 0;
         if (r < nrows) {
-          const int zq = r / ny;
+          // r / ny: a reciprocal estimate is off by at most one here
+          int zq = (int)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)ny));
+          if (zq * ny > r) --zq; else if ((zq + 1) * ny <= r) ++zq;
           const int yy = y0 + (r - zq * ny), zz = z0 + zq;
           int xa = x0, xb = x1;
           bool keep = true;
@@ -277,7 +307,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             const float rem = sh.r2max_hi - (dy * dy + dz * dz) * 0.99999f;
             keep = rem > 0.0f;
             if (keep) {
-              const float w = sqrtf(rem) * 1.00001f + 1e-3f;
+              const float w = __builtin_amdgcn_sqrtf(rem) * 1.00001f + 1e-3f;
               xa = max(xa, cell_of(sx - w, inv, gm1));
               xb = min(xb, cell_of(sx + w, inv, gm1));
             }
@@ -323,34 +353,62 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             py[u] = buf_load(ry, boff + 4u * u);
             pz[u] = buf_load(rz, boff + 4u * u);
           }
+          if constexpr (kSphere) {
+            // count pass, ascending shells: a counted pair has dist < r2max
+            // (NaN fails it); the box test only decides for the few with
+            // dist >= safe.  The shell-table reads of all SEG slots go out
+            // together, then the bin increments.
+            float dist[SEG];
+            bool hit[SEG];
 #pragma unroll
-          for (int u = 0; u < SEG; ++u) {
-            // closed, NaN-permissive box (src/objects/RTree/Mbr.c:171-182)
-            const bool in = (uint32_t)u < slen && !(px[u] < lx) && !(hx < px[u]) &&
-                            !(py[u] < ly) && !(hy < py[u]) && !(pz[u] < lz) && !(hz < pz[u]);
-            if constexpr (kInbox) {  // measurement pass: count the pair test only
-              inbox += in ? 1 : 0;
-            } else if constexpr (kShell == kShellScan) {
-              if (in) {
-                const int o = shell_off<kShell>(dpps_distance(px[u], py[u], pz[u], sx, sy, sz),
-                                                r2, lut, sh);
-                if (o >= 0) {
-                  if (kLds)
-                    atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
-                  else
-                    atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
-                                  (uint64_t)row * K + (o >> 2), 1ull);
-                }
-              }
-            } else {
-              const float dist = dpps_distance(px[u], py[u], pz[u], sx, sy, sz);
-              if (in && dist < sh.r2max) {  // the gate also rejects NaN
-                const int o = shell_off<kShell>(dist, r2, lut, sh);
+            for (int u = 0; u < SEG; ++u) {
+              dist[u] = dpps_distance(px[u], py[u], pz[u], sx, sy, sz);
+              hit[u] = ((uint32_t)u < slen) & (dist[u] < sh.r2max);
+              if (hit[u] && !(dist[u] < safe))
+                hit[u] = !(px[u] < lx) && !(hx < px[u]) && !(py[u] < ly) && !(hy < py[u]) &&
+                         !(pz[u] < lz) && !(hz < pz[u]);
+            }
+            ShellEntry e[SEG];
+#pragma unroll
+            for (int u = 0; u < SEG; ++u) {
+              // a slot without a hit reads entry 0 (its distance may be
+              // anything); a hit's bucket is below nlut
+              const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
+              e[u] = lut[hit[u] ? max(b, 0) : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < SEG; ++u) {
+              if (hit[u]) {
+                int o = (int)e[u].off + (dist[u] < e[u].t ? 0 : 4);
+                if constexpr (kShell == kShellTable)
+                  while (!(dist[u] < r2[o >> 2])) o += 4;  // ends: dist < r2[nrads-1]
                 if (kLds)
                   atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
                 else
                   atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
                                 (uint64_t)row * K + (o >> 2), 1ull);
+              }
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < SEG; ++u) {
+              // closed, NaN-permissive box (src/objects/RTree/Mbr.c:171-182)
+              const bool in = (uint32_t)u < slen && !(px[u] < lx) && !(hx < px[u]) &&
+                              !(py[u] < ly) && !(hy < py[u]) && !(pz[u] < lz) && !(hz < pz[u]);
+              if constexpr (kInbox) {  // measurement pass: count the pair test only
+                inbox += in ? 1 : 0;
+              } else {  // any thresholds: first match by a plain scan
+                if (in) {
+                  const int o = shell_off<kShell>(dpps_distance(px[u], py[u], pz[u], sx, sy, sz),
+                                                  r2, lut, sh);
+                  if (o >= 0) {
+                    if (kLds)
+                      atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
+                    else
+                      atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
+                                    (uint64_t)row * K + (o >> 2), 1ull);
+                  }
+                }
               }
             }
           }
@@ -417,7 +475,7 @@ void dispatch_group(int shell, bool lds, bool in, const IndexView& ix, const Hal
 constexpr size_t kNoLdsBytes = 4 * kCountWaves * 128;
 
 int count_group_for(int nrads, int requested) {
-  if (requested == 64 || requested == 16) {
+  if (requested == 64 || requested == 16 || requested == 8) {
     if ((size_t)nrads * kCountWaves * (64 / requested) * 4 > kMaxGroupBinBytes &&
         nrads <= kMaxLdsShells)
       return 64;
@@ -457,6 +515,8 @@ hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh
   const int shell = !sh.sorted ? kShellScan : sh.one_step ? kShellOneStep : kShellTable;
   if (g == 16)
     dispatch_group<16>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+  else if (g == 8)
+    dispatch_group<8>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
   else
     dispatch_group<64>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
   return hipGetLastError();

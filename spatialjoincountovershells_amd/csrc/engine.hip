@@ -621,8 +621,8 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     return SJCS_OK;
   }
   if (strcmp(key, "lanes_per_halo") == 0) {
-    if (value != 0 && value != 16 && value != 64)
-      return fail(e, SJCS_ERR_INVALID, "lanes_per_halo: 0, 16 or 64");
+    if (value != 0 && value != 8 && value != 16 && value != 64)
+      return fail(e, SJCS_ERR_INVALID, "lanes_per_halo: 0, 8, 16 or 64");
     e->lanes_per_halo = value;
     return SJCS_OK;
   }

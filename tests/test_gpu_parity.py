@@ -204,6 +204,27 @@ def test_cell_size_does_not_change_counts(eng, cell_milli, index_sort):
     assert_counts_equal(got, want, h)
 
 
+@pytest.mark.parametrize("lanes,segment", [(8, 4), (8, 8), (16, 4), (16, 8), (64, 4), (64, 8)])
+def test_kernel_shapes(eng, lanes, segment):
+    """Every lanes-per-halo / segment shape of the count kernel, on halos near
+    faces and corners (periodic images), with a one-step and a multi-step
+    shell table."""
+    p, h = gen_uniform(21, 40_000, 3_000, face=0.3)
+    eng.set_option("lanes_per_halo", lanes)
+    eng.set_option("segment", segment)
+    try:
+        for spec in ("0.001:5:40", "0.01:20:64", "1:600:5"):
+            mn, mx, k = t.parse_radius(spec)
+            _, r2 = t.oracle_thresholds(mn, mx, k)
+            got = run_engine(eng, [p], h, r2)
+            want, w = oracle_chunks([p], h, r2)
+            assert_counts_equal(got, want, h)
+            assert eng.last_chunk_pair_tests() == w
+    finally:
+        eng.set_option("lanes_per_halo", 0)
+        eng.set_option("segment", 0)
+
+
 @pytest.mark.parametrize("index_sort", [0, 1])
 def test_clustered_particles(eng, index_sort):
     """Most particles in one tight clump (whole sort tiles of one cell, one
