@@ -102,7 +102,10 @@ int sjcs_get_counts(sjcs_engine *e, uint64_t *counts);
 int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
 
 /* Tuning knobs (no reference counterpart).  Keys:
- *   "cell_size_milli"   cell edge in 1/1000 box units, 0 = automatic
+ *   "cell_size_milli"   cell edge along y and z in 1/1000 box units,
+ *                       0 = automatic
+ *   "x_split"           cells along x per cell along y/z (1 to 16),
+ *                       0 = automatic
  *   "kernel"            0 = automatic, 1 = one halo per wavefront (plain)
  *   "lanes_per_halo"    8, 16 or 64 lanes of a wavefront per halo, 0 =
  *                       automatic (16 up to 256 shells)

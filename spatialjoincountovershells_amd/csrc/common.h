@@ -19,14 +19,17 @@ namespace sjcs {
 constexpr float kBoxLo = 0.0f;
 constexpr float kBoxHi = 1000.0f;
 
-// Cell grid over the periodic box: G cells per axis, key = (cz*G + cy)*G + cx,
-// so the particles of one (cy, cz) row are contiguous in key order and any
-// run of x cells of a row is one contiguous index range.
+// Cell grid over the periodic box: G cells along y and z, GX along x (a
+// multiple of G: finer x cells cut a query's rows closer to its sphere), key
+// = (cz*G + cy)*GX + cx, so the particles of one (cy, cz) row are contiguous
+// in key order and any run of x cells of a row is one contiguous index range.
 struct Grid {
-  float inv_cell;  // G / 1000
-  float cell;      // 1000 / G
-  int g;           // cells per axis
-  uint32_t ncells; // g^3
+  float inv_cell;    // G / 1000
+  float cell;        // 1000 / G
+  float inv_cell_x;  // GX / 1000
+  int g;             // cells along y and z
+  int gx;            // cells along x
+  uint32_t ncells;   // g * g * gx
 };
 
 // Monotone (non-decreasing) map from a raw FP32 coordinate to a cell, clamped
@@ -54,10 +57,10 @@ __device__ __forceinline__ int cell_hi(float hi, float inv_cell, int gm1) {
 __device__ __forceinline__ uint32_t cell_key(float x, float y, float z,
                                              const Grid& gr) {
   int gm1 = gr.g - 1;
-  uint32_t cx = (uint32_t)cell_of(x, gr.inv_cell, gm1);
+  uint32_t cx = (uint32_t)cell_of(x, gr.inv_cell_x, gr.gx - 1);
   uint32_t cy = (uint32_t)cell_of(y, gr.inv_cell, gm1);
   uint32_t cz = (uint32_t)cell_of(z, gr.inv_cell, gm1);
-  return (cz * (uint32_t)gr.g + cy) * (uint32_t)gr.g + cx;
+  return (cz * (uint32_t)gr.g + cy) * (uint32_t)gr.gx + cx;
 }
 
 // Shell lookup tables for the first-match rule.  When r2 is non-decreasing,

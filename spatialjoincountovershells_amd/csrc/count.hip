@@ -205,9 +205,12 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
 
   const float rbox = sh.rbox;
   const float inv = ix.grid.inv_cell;
+  const float invx = ix.grid.inv_cell_x;
   const float cs = ix.grid.cell;
   const int G = ix.grid.g;
   const int gm1 = G - 1;
+  const int GX = ix.grid.gx;
+  const int gxm1 = GX - 1;
   uint64_t inbox = 0;
 
   for (uint64_t hb = h0 + (uint64_t)wave * H; hb < h1; hb += (uint64_t)kCountWaves * H) {
@@ -272,11 +275,11 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       }
       int x0, x1, y0, y1, z0, z1;
       if constexpr (kSphere) {
-        x0 = cell_of(lx, inv, gm1), x1 = cell_of(hx, inv, gm1);
+        x0 = cell_of(lx, invx, gxm1), x1 = cell_of(hx, invx, gxm1);
         y0 = cell_of(ly, inv, gm1), y1 = cell_of(hy, inv, gm1);
         z0 = cell_of(lz, inv, gm1), z1 = cell_of(hz, inv, gm1);
       } else {
-        x0 = cell_lo(lx, inv, gm1), x1 = cell_hi(hx, inv, gm1);
+        x0 = cell_lo(lx, invx, gxm1), x1 = cell_hi(hx, invx, gxm1);
         y0 = cell_lo(ly, inv, gm1), y1 = cell_hi(hy, inv, gm1);
         z0 = cell_lo(lz, inv, gm1), z1 = cell_hi(hz, inv, gm1);
       }
@@ -308,12 +311,12 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             keep = rem > 0.0f;
             if (keep) {
               const float w = __builtin_amdgcn_sqrtf(rem) * 1.00001f + 1e-3f;
-              xa = max(xa, cell_of(sx - w, inv, gm1));
-              xb = min(xb, cell_of(sx + w, inv, gm1));
+              xa = max(xa, cell_of(sx - w, invx, gxm1));
+              xb = min(xb, cell_of(sx + w, invx, gxm1));
             }
           }
           if (keep) {
-            const uint32_t kb = ((uint32_t)zz * (uint32_t)G + (uint32_t)yy) * (uint32_t)G;
+            const uint32_t kb = ((uint32_t)zz * (uint32_t)G + (uint32_t)yy) * (uint32_t)GX;
             const uint32_t k0 = kb + (uint32_t)xa, k1 = kb + (uint32_t)xb;
             beg = k0 ? ix.cell_end[k0 - 1] : 0u;
             len = ix.cell_end[k1] - beg;

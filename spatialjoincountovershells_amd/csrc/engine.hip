@@ -97,29 +97,38 @@ int key_bits_for(uint32_t ncells) {
   return b;
 }
 
-// Cell edge: 0.9 of the query half-width by default (a query box then spans
-// about 3 cells per axis, and its sphere cuts most of each row), coarsened
-// until there are at most ~2 cells per particle, and at most 1024 cells per
-// axis.
-Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli) {
+// Cell edge along y and z: 0.9 of the query half-width by default (a query
+// box then spans about 3 rows per axis, and its sphere cuts most of each
+// row), coarsened until there are at most ~2 cells per particle; about 8
+// particles per cell when the radius is not known.  At most 1024 cells per
+// axis.  Along x the cells are `x_split` times finer (4 by default), within
+// 1024 cells and ~4 cells per particle.
+Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli, int64_t x_split) {
   double s;
   if (cell_milli > 0)
     s = (double)cell_milli / 1000.0;
   else if (rbox > 0.0f && isfinite(rbox))
     s = (double)rbox * 0.9;
   else
-    s = 1000.0;
+    s = 1000.0 / std::max(1.0, cbrt((double)n / 8.0));
   double gd = ceil(1000.0 / s);
   if (!(gd >= 1.0)) gd = 1.0;
   if (gd > 1024.0) gd = 1024.0;
-  double cap = cbrt((double)std::max<uint64_t>(4096, 2 * n));
-  if (cell_milli <= 0 && gd > cap) gd = floor(cap);
+  const double cells_cap = (double)std::max<uint64_t>(4096, 2 * n);
+  if (cell_milli <= 0 && gd > cbrt(cells_cap)) gd = floor(cbrt(cells_cap));
   if (gd < 1.0) gd = 1.0;
+  double k = (double)(x_split > 0 ? x_split : 4);
+  const double kcap = floor(2.0 * cells_cap / (gd * gd * gd));
+  if (x_split <= 0 && k > kcap) k = std::max(1.0, kcap);
+  double gxd = std::min(1024.0, gd * k);
+  if (gxd < gd) gxd = gd;
   Grid g;
   g.g = (int)gd;
+  g.gx = (int)gxd;
   g.inv_cell = (float)g.g / 1000.0f;
   g.cell = 1000.0f / (float)g.g;
-  g.ncells = (uint32_t)g.g * (uint32_t)g.g * (uint32_t)g.g;
+  g.inv_cell_x = (float)g.gx / 1000.0f;
+  g.ncells = (uint32_t)g.g * (uint32_t)g.g * (uint32_t)g.gx;
   return g;
 }
 
@@ -162,6 +171,7 @@ struct sjcs_engine {
   int64_t lanes_per_halo = 0;
   int64_t segment = 0;
   int64_t index_sort = 0;
+  int64_t x_split = 0;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
@@ -363,7 +373,7 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   // grids of at most kCsMaxCells cells per axis use the per-axis cell sort;
   // finer ones (or "index_sort" 1) the radix sort of packed cell keys
-  const bool by_axis = g.g <= kCsMaxCells && e->index_sort != 1;
+  const bool by_axis = g.g <= kCsMaxCells && g.gx <= kCsMaxCellsX && e->index_sort != 1;
   for (int i = 0; i < 2; ++i) {
     if (!by_axis) HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.x[i], b), "hipMalloc(index)");
@@ -428,7 +438,7 @@ int lanes(const sjcs_engine* e) { return e->kernel == 1 ? 64 : (int)e->lanes_per
 int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
 
 int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
-  const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli);
+  const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli, e->x_split);
   int rc = build_index(e, e->ib, d_xyz, n, g);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
@@ -626,6 +636,11 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     e->lanes_per_halo = value;
     return SJCS_OK;
   }
+  if (strcmp(key, "x_split") == 0) {
+    if (value < 0 || value > 16) return fail(e, SJCS_ERR_INVALID, "x_split: 0 to 16");
+    e->x_split = value;
+    return SJCS_OK;
+  }
   if (strcmp(key, "index_sort") == 0) {
     if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "index_sort: 0 or 1");
     e->index_sort = value;
@@ -723,7 +738,7 @@ int sjcs_index_build(sjcs_index* ix) {
   if (rc != SJCS_OK) return rc;
   if (ix->built) return SJCS_OK;
   // no query radius is known yet: size cells for about two particles each
-  const Grid g = choose_grid(ix->n, 0.0f, e->cell_size_milli);
+  const Grid g = choose_grid(ix->n, 0.0f, e->cell_size_milli, e->x_split);
   rc = build_index(e, ix->ib, ix->aos.as<float>(), ix->n, g);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");

@@ -28,7 +28,8 @@ constexpr int kCsTile = kCsThreads * kCsItems;  // 4096 particles per block
 constexpr int kCsRowThreads = 256;
 constexpr int kCsRowItems = 16;
 constexpr int kCsRowCap = kCsRowThreads * kCsRowItems;  // 4096 particles
-constexpr int kCsMaxCells = 256;
+constexpr int kCsMaxCells = 256;    // along y and z
+constexpr int kCsMaxCellsX = 1024;  // along x
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
@@ -73,7 +74,8 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay,
 // Particle chunk (packed xyz, n < 2^32) -> SoA coordinates in cell-key
 // order in x[0], y[0], z[0] (x[1], y[1], z[1] are scratch), and the cell
 // table: cell_end[c] = number of particles in cells <= c.  Needs g.g <=
-// kCsMaxCells.  The order of the particles inside one cell is unspecified.
+// kCsMaxCells and g.gx <= kCsMaxCellsX.  The order of the particles inside
+// one cell is unspecified.
 hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
                                float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s);
 

@@ -519,6 +519,33 @@ __device__ __forceinline__ uint32_t row_begin(uint32_t rw, int G, uint64_t n,
   return ts ? hist2[(uint64_t)tb[sl] * (uint64_t)G + (uint64_t)d * ts] : ss[sl];
 }
 
+// Exclusive scan of the row's cx counts s_cnt[0 .. kCsMaxCellsX) with
+// kCsRowThreads threads, kCsMaxCellsX / kCsRowThreads consecutive digits
+// each.  Writes the row's cell table entries (start + inclusive count) and
+// leaves in s_cnt[d] the digit's first slot, plus `add`.
+__device__ __forceinline__ void cs_row_scan(uint32_t* s_cnt, uint32_t* s_tmp, uint32_t* ce,
+                                            uint32_t start, int gx, uint32_t add) {
+  constexpr int Q = kCsMaxCellsX / kCsRowThreads;
+  const int tid = threadIdx.x;
+  uint32_t c[Q], sum = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    c[q] = s_cnt[tid * Q + q];
+    sum += c[q];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan<OpSum, kCsRowThreads>(sum, s_tmp, &total);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int d = tid * Q + q;
+    if (d < gx) {
+      ce[d] = start + run + c[q];
+      s_cnt[d] = add + run;
+    }
+    run += c[q];
+  }
+}
+
 // Pass 3: every row in cx order, and the cell table.  Rows of up to
 // kCsRowCap particles are staged in LDS and written out contiguously;
 // longer ones are counted and then scattered straight from HBM.
@@ -528,18 +555,18 @@ __global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
     uint32_t* __restrict__ cell_end, uint64_t n, Grid g, const uint32_t* __restrict__ ss,
     const uint32_t* __restrict__ tb, const uint32_t* __restrict__ hist2, uint32_t nrows) {
   __shared__ float s_c[3][kCsRowCap];
-  __shared__ uint32_t s_cnt[kCsMaxCells];
+  __shared__ uint32_t s_cnt[kCsMaxCellsX];
   __shared__ uint32_t s_tmp[kCsRowThreads / 64];
   const uint32_t rw = xcd_tile();
   if (rw >= nrows) return;
-  const int G = g.g, gm1 = G - 1;
-  const float inv = g.inv_cell;
+  const int G = g.g, GX = g.gx, gxm1 = GX - 1;
+  const float invx = g.inv_cell_x;
   const int tid = threadIdx.x;
   const uint32_t start = row_begin(rw, G, n, ss, tb, hist2);
   const uint32_t len = row_begin(rw + 1, G, n, ss, tb, hist2) - start;
-  for (int i = tid; i < kCsMaxCells; i += kCsRowThreads) s_cnt[i] = 0;
+  for (int i = tid; i < kCsMaxCellsX; i += kCsRowThreads) s_cnt[i] = 0;
   __syncthreads();
-  uint32_t* ce = cell_end + (uint64_t)rw * (uint64_t)G;
+  uint32_t* ce = cell_end + (uint64_t)rw * (uint64_t)GX;
 
   if (len <= (uint32_t)kCsRowCap) {
     float px[kCsRowItems], py[kCsRowItems], pz[kCsRowItems];
@@ -558,18 +585,12 @@ __global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
     for (int j = 0; j < kCsRowItems; ++j) {
       rk[j] = 0;
       if ((uint32_t)(j * kCsRowThreads + tid) < len) {
-        const uint32_t d = (uint32_t)cell_of(px[j], inv, gm1);
+        const uint32_t d = (uint32_t)cell_of(px[j], invx, gxm1);
         rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
       }
     }
     __syncthreads();
-    const uint32_t c = s_cnt[tid];
-    uint32_t total;
-    const uint32_t ls = block_excl_scan<OpSum, kCsRowThreads>(c, s_tmp, &total);
-    if (tid < G) {
-      ce[tid] = start + ls + c;
-      s_cnt[tid] = ls;
-    }
+    cs_row_scan(s_cnt, s_tmp, ce, start, GX, 0u);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kCsRowItems; ++j) {
@@ -588,19 +609,13 @@ __global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
     }
   } else {
     for (uint32_t l = tid; l < len; l += kCsRowThreads)
-      atomicAdd(&s_cnt[cell_of(ix[start + l], inv, gm1)], 1u);
+      atomicAdd(&s_cnt[cell_of(ix[start + l], invx, gxm1)], 1u);
     __syncthreads();
-    const uint32_t c = s_cnt[tid];
-    uint32_t total;
-    const uint32_t ls = block_excl_scan<OpSum, kCsRowThreads>(c, s_tmp, &total);
-    if (tid < G) {
-      ce[tid] = start + ls + c;
-      s_cnt[tid] = start + ls;  // running output cursor of every cell
-    }
+    cs_row_scan(s_cnt, s_tmp, ce, start, GX, start);  // s_cnt: output cursor of every cell
     __syncthreads();
     for (uint32_t l = tid; l < len; l += kCsRowThreads) {
       const float x = ix[start + l], y = iy[start + l], z = iz[start + l];
-      const uint32_t p = atomicAdd(&s_cnt[cell_of(x, inv, gm1)], 1u);
+      const uint32_t p = atomicAdd(&s_cnt[cell_of(x, invx, gxm1)], 1u);
       ox[p] = x;
       oy[p] = y;
       oz[p] = z;
@@ -728,7 +743,7 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay, ui
 hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
                                float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s) {
   const int G = g.g;
-  if (G > kCsMaxCells || G < 1) return hipErrorInvalidValue;
+  if (G > kCsMaxCells || G < 1 || g.gx > kCsMaxCellsX || g.gx < 1) return hipErrorInvalidValue;
   if (n > 0xffffffffull) return hipErrorInvalidValue;
   if (n == 0) return fill_u32(cell_end, 0u, g.ncells, s);
   const uint64_t tiles1 = ceil_div(n, kCsTile);

@@ -204,6 +204,27 @@ def test_cell_size_does_not_change_counts(eng, cell_milli, index_sort):
     assert_counts_equal(got, want, h)
 
 
+@pytest.mark.parametrize("index_sort", [0, 1])
+@pytest.mark.parametrize("cell_milli,x_split", [(4500, 1), (4500, 2), (4500, 5), (4500, 16),
+                                                (2000, 7), (977, 3), (30000, 16)])
+def test_x_split_does_not_change_counts(eng, cell_milli, x_split, index_sort):
+    """Finer cells along x than along y/z (up to 1024 of them), with both
+    sorts."""
+    p, h = gen_uniform(22, 30_000, 2_000, face=0.3)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 12)
+    want, _ = oracle_chunks([p], h, r2)
+    eng.set_option("cell_size_milli", cell_milli)
+    eng.set_option("x_split", x_split)
+    eng.set_option("index_sort", index_sort)
+    try:
+        got = run_engine(eng, [p], h, r2)
+    finally:
+        eng.set_option("cell_size_milli", 0)
+        eng.set_option("x_split", 0)
+        eng.set_option("index_sort", 0)
+    assert_counts_equal(got, want, h)
+
+
 @pytest.mark.parametrize("lanes,segment", [(8, 4), (8, 8), (16, 4), (16, 8), (64, 4), (64, 8)])
 def test_kernel_shapes(eng, lanes, segment):
     """Every lanes-per-halo / segment shape of the count kernel, on halos near
