@@ -5,18 +5,21 @@ Workload (default): the reference README's search shape, one chunk --
 10M halos x 100M particles x 40 shells (``--radius 0.001:5:40``), halo and
 particle coordinates uniform FP32 in [0, 1000)^3, synthetic (seeded numpy).
 
-A step is one pass of the hot path over the resident particle chunk: cell
-keys, LSD radix sort, cell table, and the count of every halo into the u64
-counts (stored, as for the first particle file).  Inputs are in HBM before
-the timed region starts; the halos are sorted once beforehand, as the
-reference reads its halo file once before its chunk loop.
+A step is one pass of the hot path over the resident particle chunk: the
+cell sort (three MSD passes by cz, cy, cx, writing the cell table), and the
+count of every halo into the u64 counts (stored, as for the first particle
+file).  Inputs are in HBM before the timed region starts; the halos are
+sorted once beforehand, as the reference reads its halo file once before its
+chunk loop.
 
 ``value`` = W / (seconds per step), W = exact number of in-box
 (halo image, particle) pair tests of the chunk -- the reference's
 CalcDistance calls -- counted by an untimed pass.
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: the build targets one
-GPU.
+GPU.  ``--calibrate-copy MIB`` instead runs one plain device-to-device copy
+of MIB MiB (torch), the reference point for the FETCH_SIZE / WRITE_SIZE
+counters behind ``roofline.traffic``.
 """
 from __future__ import annotations
 
@@ -123,7 +126,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-halos", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy * 262144, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": a.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus}: this build runs on one GPU only "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
@@ -178,8 +191,7 @@ def main():
     count_bytes = 12 * n + 16 * nh + 8 * k * nh
     count_gbs = count_bytes / (count_ms * 1e-3) / 1e9
     count_tflops = 8 * w / (count_ms * 1e-3) / 1e12
-    # index build: packed xyz in, sorted SoA xyz out, one u32 per cell
-    g = None
+    # index build: packed xyz read once, sorted SoA xyz written once
     build_bytes = 24 * n
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     dominant = "count" if count_ms >= build_ms else "index_build"
@@ -192,12 +204,12 @@ def main():
     if dominant == "count":
         roof = {"bound": "hbm", "achieved": round(count_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(count_gbs / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "kernel": "count_plain_kernel",
+                "traffic": traffic, "kernel": "count_group_kernel",
                 "bytes_per_launch": count_bytes}
     else:
         roof = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "kernel": "index build (keys+radix sort+cell table)",
+                "traffic": traffic, "kernel": "cell sort (cs_* kernels)",
                 "bytes_per_launch": build_bytes}
 
     cpu = None
