@@ -46,9 +46,12 @@ FP32_PEAK_TFLOPS = 157.3
 
 CONFIGS = {
     "c2": dict(workload="uniform 1e8 particles x 1e7 halos, --radius 0.001:5:40, one chunk",
-               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40)),
+               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=1),
     "c1": dict(workload="uniform 1e5 particles x 1e4 halos, --radius 0.001:5:8, one chunk",
-               np=100_000, nh=10_000, radius=(0.001, 5.0, 8)),
+               np=100_000, nh=10_000, radius=(0.001, 5.0, 8), chunks=1),
+    "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
+                        "u64 counts accumulated on the device",
+               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3),
 }
 
 
@@ -56,27 +59,36 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_inputs(cfg, seed):
-    rng = np.random.default_rng(seed)
-    parts = (rng.random((cfg["np"], 3), dtype=np.float32) * np.float32(1000.0))
-    halos = (rng.random((cfg["nh"], 3), dtype=np.float32) * np.float32(1000.0))
+def uniform_points(rng, n):
+    pts = rng.random((n, 3), dtype=np.float32) * np.float32(1000.0)
     # float32 rounding of u*1000 can give exactly 1000.0: keep [0, 1000)
-    np.minimum(parts, np.nextafter(np.float32(1000.0), np.float32(0)), out=parts)
-    np.minimum(halos, np.nextafter(np.float32(1000.0), np.float32(0)), out=halos)
+    np.minimum(pts, np.nextafter(np.float32(1000.0), np.float32(0)), out=pts)
+    return pts
+
+
+def make_inputs(cfg, seed):
+    """Particle files (one array each, distinct seeds) and the halos."""
+    rng = np.random.default_rng(seed)
+    parts = [uniform_points(rng, cfg["np"])]
+    halos = uniform_points(rng, cfg["nh"])
+    for c in range(1, cfg["chunks"]):
+        parts.append(uniform_points(np.random.default_rng(seed + 1000 * c), cfg["np"]))
     return parts, halos
 
 
 def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
     """Times the reference's astr_rcount (built from its sources into
-    oracle/_ref) on the same particles and a halo sample; returns the
+    oracle/_ref) on the same particle files and a halo sample; returns the
     cpu_baseline object, or None when the binary is absent."""
     if not os.path.exists(REF_BIN):
         return None
     tmp = tempfile.mkdtemp(prefix="sjcs_cpu_")
     try:
-        pf = os.path.join(tmp, "part.dat")
+        pfs = []
+        for c, p in enumerate(parts):
+            pfs.append(os.path.join(tmp, f"part{c}.dat"))
+            p.tofile(pfs[-1])
         hf = os.path.join(tmp, "halo.list")
-        parts.tofile(pf)
         hs = halos[:nsample]
         with open(hf, "w") as f:
             for i in range(16):
@@ -87,7 +99,7 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
             f.writelines(lines)
         mn, mx, k = radius
         cmd = [REF_BIN, "--thread", str(threads), "--next-setup", str(threads),
-               "--backlog", "0", "--particle-files", pf, "--halo-file", hf,
+               "--backlog", "0", "--particle-files", *pfs, "--halo-file", hf,
                "--radius", f"{mn}:{mx}:{k}", "--output-file", os.devnull]
         t0 = time.time()
         out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
@@ -99,15 +111,18 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
         search_s = sum(int(v) for v in m) / 1000.0
         # exact W of the sample, counted on the GPU with the same floats
         eng.set_halos(hs)
-        eng.add_particles_device(d_parts, parts.shape[0])
-        w = eng.last_chunk_pair_tests()
+        w = 0
+        for d, p in zip(d_parts, parts):
+            eng.add_particles_device(d, p.shape[0])
+            w += eng.last_chunk_pair_tests()
         eng.set_halos(halos)
+        npart = sum(p.shape[0] for p in parts)
         return {"value": w / search_s, "unit": "pair-tests/s", "cores": threads,
                 "kind": "reference",
-                "sample": f"astr_rcount --thread {threads}: all {parts.shape[0]} particles x "
-                          f"{nsample} halos (first {nsample} of the same set), "
-                          f"{mn}:{mx}:{k}; W={w} in {search_s:.3f} s of 'searching time' "
-                          f"(process wall {wall:.1f} s)"}
+                "sample": f"astr_rcount --thread {threads}: all {npart} particles "
+                          f"({len(parts)} file(s)) x {nsample} halos (first {nsample} of the "
+                          f"same set), {mn}:{mx}:{k}; W={w} in {search_s:.3f} s of "
+                          f"'searching time' (process wall {wall:.1f} s)"}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -123,6 +138,8 @@ def main():
     ap.add_argument("--lanes-per-halo", type=int, default=0)
     ap.add_argument("--plain", action="store_true",
                     help="one halo per wavefront count kernel")
+    ap.add_argument("--overlap", action="store_true",
+                    help="sort the next chunk while this one is counted")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-halos", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -155,40 +172,52 @@ def main():
         eng.set_option("lanes_per_halo", args.lanes_per_halo)
     if args.plain:
         eng.set_option("kernel", 1)
+    if args.overlap:
+        eng.set_option("overlap", 1)
     eng.set_shells(r2)
     eng.set_halos(halos)
-    d_parts = eng.device_array(parts)
-    n = parts.shape[0]
+    d_parts = [eng.device_array(p) for p in parts]
+    n = cfg["np"]
+
+    def step():
+        eng.reset_counts()
+        for d in d_parts:
+            eng.add_particles_device(d, n)
+
+    # exact W of one step: an untimed pair-test pass per chunk
+    w = 0
+    for d in d_parts:
+        eng.reset_counts()
+        eng.add_particles_device(d, n)
+        w += eng.last_chunk_pair_tests()
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
-        eng.reset_counts()
-        eng.add_particles_device(d_parts, n)
+        step()
         eng.sync()
 
     log(f"[bench] timing {args.steps} steps")
     phase = np.zeros((args.steps, 3))
     eng.sync()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        eng.reset_counts()
-        eng.add_particles_device(d_parts, n)
-        phase[s] = eng.last_chunk_timings()  # waits for the step
+    for s_ in range(args.steps):
+        step()
+        phase[s_] = eng.last_chunk_timings()  # waits for the step (last chunk's phases)
     eng.sync()
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1000.0 / args.steps
 
-    w = eng.last_chunk_pair_tests()
     counted = int(eng.counts().sum())
     build_ms = float(phase[:, 1].mean())
     count_ms = float(phase[:, 2].mean())
     log(f"[bench] W={w} counted={counted} step={ms_step:.3f} ms build={build_ms:.3f} "
-        f"count={count_ms:.3f} ms")
+        f"count={count_ms:.3f} ms (last chunk)")
 
     nh = halos.shape[0]
     # algorithmic bytes of one count launch: particles read once, halo SoA
-    # (x, y, z, row) read once, counts written once (u64)
-    count_bytes = 12 * n + 16 * nh + 8 * k * nh
+    # x, y, z read once, u32 running counts written once (read too when a
+    # later chunk adds into them)
+    count_bytes = 12 * n + 12 * nh + 4 * k * nh * (2 if cfg["chunks"] > 1 else 1)
     count_gbs = count_bytes / (count_ms * 1e-3) / 1e9
     count_tflops = 8 * w / (count_ms * 1e-3) / 1e12
     # index build: packed xyz read once, sorted SoA xyz written once
@@ -213,16 +242,19 @@ def main():
                 "bytes_per_launch": build_bytes}
 
     cpu = None
-    if not args.no_cpu_baseline and args.config == "c2":
+    if not args.no_cpu_baseline and args.config in ("c2", "c3"):
         log("[bench] cpu baseline (reference astr_rcount on a halo sample)")
         cpu = cpu_baseline(parts, halos, cfg["radius"], eng, d_parts,
                            min(args.cpu_halos, nh), args.cpu_threads)
 
-    eng.device_free(d_parts)
+    for d in d_parts:
+        eng.device_free(d)
     eng.close()
     value = w / (ms_step * 1e-3)
     out = {
-        "metric": "shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)",
+        "metric": ("shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)"
+                   if args.config != "c3" else
+                   "shell pair-tests/s (10M halos x 3 x 100M particles x 40 shells, 1 GPU)"),
         "value": value,
         "unit": "pair-tests/s",
         "n_gpus": 1,
@@ -234,10 +266,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"], "particles": n, "halos": nh, "shells": k,
+        "config": {"workload": cfg["workload"], "particles": n * cfg["chunks"],
+                   "chunks": cfg["chunks"], "halos": nh, "shells": k,
                    "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
                    "pairs_counted": counted, "parallelism": "single GPU"},
-        "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4)},
+        "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4),
+                      "of": "the last chunk of a step" if cfg["chunks"] > 1 else "the chunk"},
         "count_kernel": {"pair_tests_per_s": w / (count_ms * 1e-3),
                          "valu_tflops_8_per_test": round(count_tflops, 3),
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),

@@ -111,6 +111,12 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *                       automatic (16 up to 256 shells)
  *   "segment"           particles of a cell row one lane walks: 4 or 8,
  *                       0 = automatic
+ *   "fold_limit"        bound on a running u32 count before it is folded
+ *                       into the u64 totals (0 = 2^32-1; small values
+ *                       exercise the folds)
+ *   "overlap"           1 = sort chunk i+1 while chunk i is counted
+ *                       (0, default: the upload still overlaps, the sort
+ *                       waits)
  *   "index_sort"        0 = automatic (axis by axis up to 256 cells per
  *                       axis), 1 = always by the packed cell key
  * Returns SJCS_ERR_INVALID for an unknown key or value. */

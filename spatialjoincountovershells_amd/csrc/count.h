@@ -23,13 +23,23 @@ int count_group_for(int nrads, int requested);
 // Dynamic LDS of the count kernel for nrads shells at `group` lanes/halo.
 size_t count_lds_bytes(int nrads, int group);
 
-// counts[row*nrads + k] (= or +=) the shell-k count of every halo against
-// the indexed particles, `group` lanes per halo (64: one halo per
-// wavefront; 0: automatic), lanes walking `seg` (4 or 8) particles of a row
-// each.  With `inbox` not null, nothing is counted and inbox receives += the
-// number of in-box pair tests.  No allocation.
+// counts[i*nrads + k] (= or +=) the shell-k count of sorted halo i against
+// the indexed particles (u32: the caller keeps a running total below 2^32,
+// one particle counting at most once per periodic image), `group` lanes per
+// halo (64: one halo per wavefront; 0: automatic), lanes walking `seg` (4 or
+// 8) particles of a row each.  With `inbox` not null, nothing is counted and
+// inbox receives += the number of in-box pair tests.  No allocation.
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
-                       uint64_t* counts, bool accumulate, unsigned long long* inbox,
+                       uint32_t* counts, bool accumulate, unsigned long long* inbox,
                        int group, int seg, hipStream_t s);
+
+// c64[i] = (store ? 0 : c64[i]) + c32[i] for i < total.
+hipError_t widen_counts(const uint32_t* c32, uint64_t* c64, uint64_t total, bool store,
+                        hipStream_t s);
+
+// dst[orig[i]*K + k] = c64[i*K + k] + c32[i*K + k] (a null source adds 0):
+// counts from sorted halo order back to the halos' input order.
+hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* orig,
+                          uint64_t* dst, uint64_t n, int K, hipStream_t s);
 
 }  // namespace sjcs

@@ -157,7 +157,7 @@ __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t byt
 
 template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox>
 __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
-    IndexView ix, HaloView hv, Shells sh, uint64_t* __restrict__ counts,
+    IndexView ix, HaloView hv, Shells sh, uint32_t* __restrict__ counts,
     uint32_t halos_per_block, int accumulate, unsigned long long* __restrict__ inbox_total) {
   constexpr int H = 64 / GROUP;
   // Sorted shells: thresholds are finite or +inf, so a live halo's boxes are
@@ -217,12 +217,14 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     const uint64_t hs = hb + grp;
     const bool active = hs < h1;
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    uint32_t row = 0;
+    // counts are kept in the sorted halo order (rows of neighbouring halos
+    // are neighbours in memory); they are put back in input order once, when
+    // they are read out
+    const uint64_t row = hs;
     if (active) {
       cx = hv.x[hs];
       cy = hv.y[hs];
       cz = hv.z[hs];
-      row = hv.orig[hs];
     }
     if (kLds && !kInbox) {
       for (int k = gl; k < K; k += GROUP) bins[k] = 0u;
@@ -388,8 +390,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
                 if (kLds)
                   atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
                 else
-                  atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
-                                (uint64_t)row * K + (o >> 2), 1ull);
+                  atomicAdd(counts + (uint64_t)row * K + (o >> 2), 1u);
               }
             }
           } else {
@@ -408,8 +409,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
                     if (kLds)
                       atomicAdd(reinterpret_cast<uint32_t*>(bins_bytes + o), 1u);
                     else
-                      atomicAdd(reinterpret_cast<unsigned long long*>(counts) +
-                                    (uint64_t)row * K + (o >> 2), 1ull);
+                      atomicAdd(counts + (uint64_t)row * K + (o >> 2), 1u);
                   }
                 }
               }
@@ -422,9 +422,9 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     if (kLds && !kInbox) {
       wave_sync();
       if (active) {
-        uint64_t* dst = counts + (uint64_t)row * K;
+        uint32_t* dst = counts + (uint64_t)row * K;
         for (int k = gl; k < K; k += GROUP) {
-          const uint64_t v = bins[k];
+          const uint32_t v = bins[k];
           if (accumulate) dst[k] += v; else dst[k] = v;
         }
       }
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
 
 template <int GROUP, int kShell, bool kLds, bool kInbox>
 void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh,
-                  uint64_t* counts, int accumulate, unsigned long long* inbox,
+                  uint32_t* counts, int accumulate, unsigned long long* inbox,
                   uint32_t nblocks, size_t lds, int seg, hipStream_t s) {
   if (seg >= 8)
     count_group_kernel<GROUP, 8, kShell, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
@@ -453,7 +453,7 @@ void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh,
 
 template <int GROUP, int kShell>
 void dispatch_flags(bool lds, bool in, const IndexView& ix, const HaloView& hv,
-                    const Shells& sh, uint64_t* counts, int acc, unsigned long long* inbox,
+                    const Shells& sh, uint32_t* counts, int acc, unsigned long long* inbox,
                     uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
   if (lds && !in) launch_group<GROUP, kShell, true, false>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
   else if (lds && in) launch_group<GROUP, kShell, true, true>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
@@ -463,7 +463,7 @@ void dispatch_flags(bool lds, bool in, const IndexView& ix, const HaloView& hv,
 
 template <int GROUP>
 void dispatch_group(int shell, bool lds, bool in, const IndexView& ix, const HaloView& hv,
-                    const Shells& sh, uint64_t* counts, int acc, unsigned long long* inbox,
+                    const Shells& sh, uint32_t* counts, int acc, unsigned long long* inbox,
                     uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
   if (shell == kShellOneStep)
     dispatch_flags<GROUP, kShellOneStep>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
@@ -473,7 +473,52 @@ void dispatch_group(int shell, bool lds, bool in, const IndexView& ix, const Hal
     dispatch_flags<GROUP, kShellScan>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
 }
 
+// c64[i] (= or +=) c32[i]: the chunk counts folded into the u64 totals.
+__global__ void widen_counts_kernel(const uint32_t* __restrict__ c32, uint64_t* __restrict__ c64,
+                                    uint64_t total, int store) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c64[i] = (store ? 0ull : c64[i]) + c32[i];
+}
+
+// dst[orig[i] * K + k] = c64[i * K + k] + c32[i * K + k] (either may be
+// absent): rows back in input order, one row per wavefront at a time.
+__global__ void readout_counts_kernel(const uint32_t* __restrict__ c32,
+                                      const uint64_t* __restrict__ c64,
+                                      const uint32_t* __restrict__ orig, uint64_t* __restrict__ dst,
+                                      uint64_t n, uint32_t K) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  const int lane = threadIdx.x & 63;
+  for (uint64_t i = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n;
+       i += waves) {
+    uint64_t* d = dst + (uint64_t)orig[i] * K;
+    for (uint32_t k = lane; k < K; k += 64) {
+      const uint64_t a = c64 ? c64[i * K + k] : 0ull;
+      const uint64_t b = c32 ? c32[i * K + k] : 0u;
+      d[k] = a + b;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t widen_counts(const uint32_t* c32, uint64_t* c64, uint64_t total, bool store,
+                        hipStream_t s) {
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = ceil_div(total, 256);
+  if (blocks > 65536) blocks = 65536;
+  widen_counts_kernel<<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, total, store ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* orig,
+                          uint64_t* dst, uint64_t n, int K, hipStream_t s) {
+  if (n == 0 || K <= 0) return hipSuccess;
+  uint64_t blocks = ceil_div(n, 4);
+  if (blocks > 65536) blocks = 65536;
+  readout_counts_kernel<<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, orig, dst, n, (uint32_t)K);
+  return hipGetLastError();
+}
 
 constexpr size_t kNoLdsBytes = 4 * kCountWaves * 128;
 
@@ -495,7 +540,7 @@ size_t count_lds_bytes(int nrads, int group) {
 }
 
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
-                       uint64_t* counts, bool accumulate, unsigned long long* inbox,
+                       uint32_t* counts, bool accumulate, unsigned long long* inbox,
                        int group, int seg, hipStream_t s) {
   if (hv.n == 0 || sh.nrads <= 0) return hipSuccess;
   const bool lds = sh.nrads <= kMaxLdsShells;
@@ -503,7 +548,7 @@ hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh
   if (!measure && !accumulate && (!lds || ix.n == 0)) {
     // the HBM-atomic path adds into the counts, and an empty chunk writes
     // nothing: either way a first chunk must start from zero
-    hipError_t e = fill_u64(counts, 0ull, hv.n * (uint64_t)sh.nrads, s);
+    hipError_t e = fill_u32(counts, 0u, hv.n * (uint64_t)sh.nrads, s);
     if (e != hipSuccess) return e;
   }
   if (ix.n == 0) return hipSuccess;

@@ -148,7 +148,8 @@ float bits_float(uint32_t b) {
 
 struct sjcs_engine {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // counts, halo sorts, the index API
+  hipStream_t bstream = nullptr;  // chunk uploads and cell sorts
   std::string err;
 
   bool have_shells = false;
@@ -156,14 +157,34 @@ struct sjcs_engine {
   bool have_halos = false;
   HaloState halos;
 
-  DevBuf counts;
-  bool counts_fresh = true;  // no chunk since the last reset: next one stores
+  // Counts, rows in sorted halo order: u32 running counts of the chunks
+  // since the last fold, and u64 totals they are folded into before any
+  // count could reach 2^32 (a particle counts at most once per image).
+  DevBuf counts;    // u32 [nhalos][K]
+  DevBuf counts64;  // u64 [nhalos][K]
+  DevBuf ucounts;   // u64, input order, for read-out
+  bool counts_fresh = true;  // the u32 counts hold nothing: next chunk stores
+  bool c64_live = false;     // counts64 holds totals
+  uint64_t pending = 0;      // bound on any u32 count
 
-  IndexBuffers ib;
-  DevBuf in;  // AoS chunk staging on the device
+
+  // Chunk i uses slot i % 2: its upload (bstream) overlaps the count of
+  // chunk i - 1 (stream).  Its cell sort waits for that count ("overlap" 0,
+  // the default: sort and count kernels would compete for the same CUs) or
+  // only for the count that last read the slot ("overlap" 1).
+  IndexBuffers ibs[2];
+  DevBuf cin[2];              // packed chunk staged on the device (host uploads)
+  void* cpin[2] = {nullptr, nullptr};  // pinned host staging of a chunk
+  size_t cpin_bytes[2] = {0, 0};
+  int slot = 0;               // slot of the next chunk
+  int last = -1;              // slot of the last chunk
+  hipEvent_t built[2] = {nullptr, nullptr};     // slot's sort done
+  hipEvent_t counted[2] = {nullptr, nullptr};   // slot's count done
+  hipEvent_t uploaded[2] = {nullptr, nullptr};  // slot's pinned buffer read
+  DevBuf in;  // halo / query staging on the device
   DevBuf ws;  // sort / scan workspace
   DevBuf inbox;
-  void* pinned = nullptr;
+  void* pinned = nullptr;  // pinned staging of halos / queries
   size_t pinned_bytes = 0;
 
   int64_t cell_size_milli = 0;
@@ -172,8 +193,11 @@ struct sjcs_engine {
   int64_t segment = 0;
   int64_t index_sort = 0;
   int64_t x_split = 0;
+  int64_t overlap = 0;
+  int64_t fold_limit = 0;  // 0: fold before a u32 count could pass 2^32 - 1
 
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // last chunk: upload start/end, sort start/end, count start/end
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
 };
 
@@ -209,10 +233,18 @@ int use_device(sjcs_engine* e) {
   return SJCS_OK;
 }
 
-// Pinned host staging of at least `bytes`; waits for the stream first so a
+// Waits for all work of the engine (both streams).
+int quiesce(sjcs_engine* e) {
+  HIPCALL(e, hipStreamSynchronize(e->bstream), "hipStreamSynchronize");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  return SJCS_OK;
+}
+
+// Pinned host staging of at least `bytes`; waits for all work first so a
 // previous upload from it has finished.
 int ensure_pinned(sjcs_engine* e, size_t bytes) {
-  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  int rc = quiesce(e);
+  if (rc != SJCS_OK) return rc;
   if (e->pinned_bytes >= bytes) return SJCS_OK;
   if (e->pinned) (void)hipHostFree(e->pinned);
   e->pinned = nullptr;
@@ -367,22 +399,35 @@ int upload_points(sjcs_engine* e, DevBuf& dst, const float* pts, size_t stride, 
   return SJCS_OK;
 }
 
-// Builds the cell index of n particles at d_xyz (device, packed) into ib.
-int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g) {
+// Builds the cell index of n particles at d_xyz (device, packed) into ib,
+// on stream st.  Buffers that must grow are reallocated only after all work
+// of the engine is done.
+int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
+                hipStream_t st) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles per chunk");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   // grids of at most kCsMaxCells cells per axis use the per-axis cell sort;
   // finer ones (or "index_sort" 1) the radix sort of packed cell keys
   const bool by_axis = g.g <= kCsMaxCells && g.gx <= kCsMaxCellsX && e->index_sort != 1;
+  const uint64_t wsw = by_axis ? cell_sort_workspace_words(n)
+                               : std::max(radix_workspace_words(n), scan_workspace_words(g.ncells));
+  const size_t ceb = sizeof(uint32_t) * (size_t)g.ncells;
+  bool grow = e->ws.bytes < wsw * 4 || ib.cell_end.bytes < ceb;
+  for (int i = 0; i < 2; ++i)
+    grow = grow || (!by_axis && ib.key[i].bytes < b) || ib.x[i].bytes < b ||
+           ib.y[i].bytes < b || ib.z[i].bytes < b;
+  if (grow) {
+    int rc = quiesce(e);
+    if (rc != SJCS_OK) return rc;
+  }
   for (int i = 0; i < 2; ++i) {
     if (!by_axis) HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.x[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.y[i], b), "hipMalloc(index)");
     HIPCALL(e, ensure(ib.z[i], b), "hipMalloc(index)");
   }
-  HIPCALL(e, ensure(ib.cell_end, sizeof(uint32_t) * (size_t)g.ncells), "hipMalloc(cell table)");
-  int rc = ensure_ws(e, by_axis ? cell_sort_workspace_words(n)
-                                : std::max(radix_workspace_words(n), scan_workspace_words(g.ncells)));
+  HIPCALL(e, ensure(ib.cell_end, ceb), "hipMalloc(cell table)");
+  int rc = ensure_ws(e, wsw);
   if (rc != SJCS_OK) return rc;
   ib.grid = g;
   ib.n = n;
@@ -391,25 +436,25 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n
     float* ys[2] = {ib.y[0].as<float>(), ib.y[1].as<float>()};
     float* zs[2] = {ib.z[0].as<float>(), ib.z[1].as<float>()};
     HIPCALL(e, cell_sort_particles(d_xyz, n, g, xs, ys, zs, ib.cell_end.as<uint32_t>(),
-                                   e->ws.as<uint32_t>(), e->stream),
+                                   e->ws.as<uint32_t>(), st),
             "cell sort");
     ib.sel = 0;
     return SJCS_OK;
   }
   HIPCALL(e, particle_keys(d_xyz, n, g, ib.key[0].as<uint32_t>(), ib.x[0].as<float>(),
-                           ib.y[0].as<float>(), ib.z[0].as<float>(), e->stream),
+                           ib.y[0].as<float>(), ib.z[0].as<float>(), st),
           "cell keys");
   uint32_t* keys[2] = {ib.key[0].as<uint32_t>(), ib.key[1].as<uint32_t>()};
   uint32_t* pay[2][4] = {
       {ib.x[0].as<uint32_t>(), ib.y[0].as<uint32_t>(), ib.z[0].as<uint32_t>(), nullptr},
       {ib.x[1].as<uint32_t>(), ib.y[1].as<uint32_t>(), ib.z[1].as<uint32_t>(), nullptr}};
   int sel = 0;
-  HIPCALL(e, radix_sort_pairs(keys, pay, 3, n, key_bits_for(g.ncells), e->ws.as<uint32_t>(),
-                              e->stream, &sel),
+  HIPCALL(e, radix_sort_pairs(keys, pay, 3, n, key_bits_for(g.ncells), e->ws.as<uint32_t>(), st,
+                              &sel),
           "radix sort");
   ib.sel = sel;
   HIPCALL(e, build_cell_end(ib.key[sel].as<uint32_t>(), n, ib.cell_end.as<uint32_t>(), g.ncells,
-                            e->ws.as<uint32_t>(), e->stream),
+                            e->ws.as<uint32_t>(), st),
           "cell table");
   return SJCS_OK;
 }
@@ -426,8 +471,14 @@ HaloView halo_view(const HaloState& h) {
 
 int prepare_counts(sjcs_engine* e) {
   e->counts_fresh = true;
+  e->c64_live = false;
+  e->pending = 0;
   if (!e->have_shells || !e->have_halos) return SJCS_OK;
-  const size_t bytes = sizeof(uint64_t) * (size_t)e->halos.n * (size_t)e->shells.nrads;
+  const size_t bytes = sizeof(uint32_t) * (size_t)e->halos.n * (size_t)e->shells.nrads;
+  if (e->counts.bytes < bytes) {
+    int rc = quiesce(e);
+    if (rc != SJCS_OK) return rc;
+  }
   HIPCALL(e, ensure(e->counts, bytes), "hipMalloc(counts)");
   return SJCS_OK;
 }
@@ -437,18 +488,75 @@ int prepare_counts(sjcs_engine* e) {
 int lanes(const sjcs_engine* e) { return e->kernel == 1 ? 64 : (int)e->lanes_per_halo; }
 int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
 
-int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+// Most images of one halo a particle can lie in: 1 unless the query box is
+// about as wide as the periodic box (or the radius is not finite).
+uint64_t images_max(const sjcs_engine* e) {
+  const float rb = e->shells.view.rbox;
+  return rb < 499.0f ? 1u : 8u;
+}
+
+// Folds the u32 running counts into the u64 totals (stream).
+int fold_counts(sjcs_engine* e) {
+  const uint64_t total = e->halos.n * (uint64_t)e->shells.nrads;
+  if (e->counts64.bytes < total * sizeof(uint64_t)) {
+    int rc = quiesce(e);
+    if (rc != SJCS_OK) return rc;
+    HIPCALL(e, ensure(e->counts64, total * sizeof(uint64_t)), "hipMalloc(counts)");
+  }
+  HIPCALL(e, widen_counts(e->counts.as<uint32_t>(), e->counts64.as<uint64_t>(), total,
+                          !e->c64_live, e->stream),
+          "fold counts");
+  e->c64_live = true;
+  e->counts_fresh = true;
+  e->pending = 0;
+  return SJCS_OK;
+}
+
+// Sorts one piece into the current slot (bstream) and counts it (stream).
+int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+  const int sl = e->slot;
+  const uint64_t img = images_max(e);
+  const uint64_t limit = e->fold_limit > 0 ? (uint64_t)e->fold_limit : 0xffffffffull;
+  if (!e->counts_fresh && e->pending + img * n > limit) {
+    int rc = fold_counts(e);
+    if (rc != SJCS_OK) return rc;
+  }
   const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli, e->x_split);
-  int rc = build_index(e, e->ib, d_xyz, n, g);
+  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+  if (!e->overlap && e->last >= 0)
+    HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[e->last], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipEventRecord(e->ev[2], e->bstream), "hipEventRecord");
+  int rc = build_index(e, e->ibs[sl], d_xyz, n, g, e->bstream);
   if (rc != SJCS_OK) return rc;
-  HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
-  HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view,
-                         e->counts.as<uint64_t>(), !e->counts_fresh, nullptr, lanes(e),
+  HIPCALL(e, hipEventRecord(e->ev[3], e->bstream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->built[sl], e->bstream), "hipEventRecord");
+  HIPCALL(e, hipStreamWaitEvent(e->stream, e->built[sl], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
+  HIPCALL(e, count_halos(e->ibs[sl].view(), halo_view(e->halos), e->shells.view,
+                         e->counts.as<uint32_t>(), !e->counts_fresh, nullptr, lanes(e),
                          seg_len(e), e->stream),
           "count kernel");
-  HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->counted[sl], e->stream), "hipEventRecord");
+  e->last = sl;
+  e->slot = sl ^ 1;
   e->counts_fresh = false;
+  e->pending += img * n;
   e->timed = true;
+  return SJCS_OK;
+}
+
+// A chunk, in pieces small enough for 32-bit particle offsets and for
+// per-piece counts below 2^32.
+int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+  const uint64_t cap = std::min<uint64_t>(0x3fffffffull, 0xffffffffull / images_max(e));
+  uint64_t done = 0;
+  do {
+    const uint64_t m = std::min(cap, n - done);
+    int rc = add_piece(e, d_xyz + 3 * done, m);
+    if (rc != SJCS_OK) return rc;
+    done += m;
+  } while (done < n);
   return SJCS_OK;
 }
 
@@ -480,7 +588,17 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
   e->device = device;
   h = hipSetDevice(device);
   if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
-  for (int i = 0; i < 4 && h == hipSuccess; ++i) h = hipEventCreate(&e->ev[i]);
+  if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->bstream, hipStreamNonBlocking);
+  for (int i = 0; i < 6 && h == hipSuccess; ++i) h = hipEventCreate(&e->ev[i]);
+  for (int i = 0; i < 2 && h == hipSuccess; ++i) {
+    h = hipEventCreateWithFlags(&e->built[i], hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->counted[i], hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->uploaded[i], hipEventDisableTiming);
+    // recorded once so that the first waits on them return at once
+    if (h == hipSuccess) h = hipEventRecord(e->built[i], e->stream);
+    if (h == hipSuccess) h = hipEventRecord(e->counted[i], e->stream);
+    if (h == hipSuccess) h = hipEventRecord(e->uploaded[i], e->stream);
+  }
   if (h == hipSuccess) h = ensure(e->inbox, sizeof(unsigned long long));
   if (h != hipSuccess) {
     g_create_error = std::string("engine setup: ") + hipGetErrorString(h);
@@ -494,6 +612,7 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
 void sjcs_engine_destroy(sjcs_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
+  if (e->bstream) (void)hipStreamSynchronize(e->bstream);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   e->shells.d_r2.release();
   e->shells.d_lut.release();
@@ -502,13 +621,23 @@ void sjcs_engine_destroy(sjcs_engine* e) {
   e->halos.z.release();
   e->halos.row.release();
   e->counts.release();
-  e->ib.release();
+  e->counts64.release();
+  e->ucounts.release();
+  for (int i = 0; i < 2; ++i) {
+    e->ibs[i].release();
+    e->cin[i].release();
+    if (e->cpin[i]) (void)hipHostFree(e->cpin[i]);
+    if (e->built[i]) (void)hipEventDestroy(e->built[i]);
+    if (e->counted[i]) (void)hipEventDestroy(e->counted[i]);
+    if (e->uploaded[i]) (void)hipEventDestroy(e->uploaded[i]);
+  }
   e->in.release();
   e->ws.release();
   e->inbox.release();
   if (e->pinned) (void)hipHostFree(e->pinned);
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 6; ++i)
     if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  if (e->bstream) (void)hipStreamDestroy(e->bstream);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -520,6 +649,7 @@ const char* sjcs_last_error(const sjcs_engine* e) {
 int sjcs_set_shells(sjcs_engine* e, const float* radiuses_sq, int nrads) {
   if (!e) return SJCS_ERR_INVALID;
   int rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the shells
   if (rc != SJCS_OK) return rc;
   rc = make_shells(e, e->shells, radiuses_sq, nrads);
   if (rc != SJCS_OK) return rc;
@@ -532,6 +662,7 @@ int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uin
   if ((!points && nhalos) || stride_bytes < 12)
     return fail(e, SJCS_ERR_INVALID, "halo points: null pointer or stride < 12");
   int rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   rc = upload_points(e, e->in, points, stride_bytes, nhalos);
   if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos);
@@ -544,6 +675,7 @@ int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
   if (!e) return SJCS_ERR_INVALID;
   if (!d_xyz && nhalos) return fail(e, SJCS_ERR_INVALID, "halo points: null pointer");
   int rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   rc = sort_halos(e, e->halos, d_xyz, nhalos);
   if (rc != SJCS_OK) return rc;
@@ -560,19 +692,37 @@ int sjcs_add_particles(sjcs_engine* e, const float* xyz, uint64_t n) {
   int rc = check_ready(e);
   if (rc != SJCS_OK) return rc;
   if (!xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
-  HIPCALL(e, hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
-  rc = upload_points(e, e->in, xyz, 12, n);
-  if (rc != SJCS_OK) return rc;
-  HIPCALL(e, hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
-  return add_chunk(e, e->in.as<float>(), n);
+  const int sl = e->slot;
+  const size_t bytes = 12 * (size_t)(n ? n : 1);
+  // the slot's pinned buffer is free once its last upload is done; its
+  // device staging once the sort that read it is done
+  HIPCALL(e, hipEventSynchronize(e->uploaded[sl]), "hipEventSynchronize");
+  if (e->cpin_bytes[sl] < bytes) {
+    if (e->cpin[sl]) (void)hipHostFree(e->cpin[sl]);
+    e->cpin[sl] = nullptr;
+    e->cpin_bytes[sl] = 0;
+    HIPCALL(e, hipHostMalloc(&e->cpin[sl], bytes, hipHostMallocDefault), "hipHostMalloc");
+    e->cpin_bytes[sl] = bytes;
+  }
+  if (e->cin[sl].bytes < bytes) HIPCALL(e, hipEventSynchronize(e->built[sl]), "hipEventSynchronize");
+  HIPCALL(e, ensure(e->cin[sl], bytes), "hipMalloc(points)");
+  if (n) memcpy(e->cpin[sl], xyz, 12 * (size_t)n);
+  HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
+  if (n)
+    HIPCALL(e, hipMemcpyAsync(e->cin[sl].p, e->cpin[sl], 12 * (size_t)n, hipMemcpyHostToDevice,
+                              e->bstream),
+            "hipMemcpy(points)");
+  HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
+  return add_chunk(e, e->cin[sl].as<float>(), n);
 }
 
 int sjcs_add_particles_device(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   int rc = check_ready(e);
   if (rc != SJCS_OK) return rc;
   if (!d_xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
-  HIPCALL(e, hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
-  HIPCALL(e, hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
   return add_chunk(e, d_xyz, n);
 }
 
@@ -580,8 +730,7 @@ int sjcs_sync(sjcs_engine* e) {
   if (!e) return SJCS_ERR_INVALID;
   int rc = use_device(e);
   if (rc != SJCS_OK) return rc;
-  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-  return SJCS_OK;
+  return quiesce(e);
 }
 
 int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
@@ -589,14 +738,23 @@ int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
   if (rc != SJCS_OK) return rc;
   const size_t n = (size_t)e->halos.n * (size_t)e->shells.nrads;
   if (!counts && n) return fail(e, SJCS_ERR_INVALID, "counts: null pointer");
-  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  rc = quiesce(e);
+  if (rc != SJCS_OK) return rc;
   if (n == 0) return SJCS_OK;
-  if (e->counts_fresh) {
+  if (e->counts_fresh && !e->c64_live) {
     memset(counts, 0, n * sizeof(uint64_t));
     return SJCS_OK;
   }
-  HIPCALL(e, hipMemcpy(counts, e->counts.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost),
+  HIPCALL(e, ensure(e->ucounts, n * sizeof(uint64_t)), "hipMalloc(counts)");
+  HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
+                            e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
+                            e->halos.row.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
+                            e->shells.nrads, e->stream),
+          "read out counts");
+  HIPCALL(e, hipMemcpyAsync(counts, e->ucounts.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            e->stream),
           "hipMemcpy(counts)");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
   return SJCS_OK;
 }
 
@@ -607,7 +765,7 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
   *pair_tests = 0;
   if (!e->timed) return SJCS_OK;
   HIPCALL(e, fill_u64(e->inbox.as<uint64_t>(), 0ull, 1, e->stream), "fill");
-  HIPCALL(e, count_halos(e->ib.view(), halo_view(e->halos), e->shells.view, nullptr, true,
+  HIPCALL(e, count_halos(e->ibs[e->last].view(), halo_view(e->halos), e->shells.view, nullptr, true,
                          e->inbox.as<unsigned long long>(), lanes(e), seg_len(e), e->stream),
           "pair-test count");
   HIPCALL(e, hipMemcpyAsync(pair_tests, e->inbox.p, 8, hipMemcpyDeviceToHost, e->stream),
@@ -636,6 +794,16 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     e->lanes_per_halo = value;
     return SJCS_OK;
   }
+  if (strcmp(key, "fold_limit") == 0) {
+    if (value < 0 || value > 0xffffffffll) return fail(e, SJCS_ERR_INVALID, "fold_limit: 0 to 2^32-1");
+    e->fold_limit = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "overlap") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "overlap: 0 or 1");
+    e->overlap = value;
+    return SJCS_OK;
+  }
   if (strcmp(key, "x_split") == 0) {
     if (value < 0 || value > 16) return fail(e, SJCS_ERR_INVALID, "x_split: 0 to 16");
     e->x_split = value;
@@ -660,10 +828,11 @@ int sjcs_last_chunk_timings(sjcs_engine* e, float ms[3]) {
   if (!e->timed) return SJCS_OK;
   int rc = use_device(e);
   if (rc != SJCS_OK) return rc;
-  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  rc = quiesce(e);
+  if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventElapsedTime(&ms[0], e->ev[0], e->ev[1]), "hipEventElapsedTime");
-  HIPCALL(e, hipEventElapsedTime(&ms[1], e->ev[1], e->ev[2]), "hipEventElapsedTime");
-  HIPCALL(e, hipEventElapsedTime(&ms[2], e->ev[2], e->ev[3]), "hipEventElapsedTime");
+  HIPCALL(e, hipEventElapsedTime(&ms[1], e->ev[2], e->ev[3]), "hipEventElapsedTime");
+  HIPCALL(e, hipEventElapsedTime(&ms[2], e->ev[4], e->ev[5]), "hipEventElapsedTime");
   return SJCS_OK;
 }
 
@@ -739,7 +908,9 @@ int sjcs_index_build(sjcs_index* ix) {
   if (ix->built) return SJCS_OK;
   // no query radius is known yet: size cells for about two particles each
   const Grid g = choose_grid(ix->n, 0.0f, e->cell_size_milli, e->x_split);
-  rc = build_index(e, ix->ib, ix->aos.as<float>(), ix->n, g);
+  rc = quiesce(e);  // the sort workspace is shared with the chunk path
+  if (rc != SJCS_OK) return rc;
+  rc = build_index(e, ix->ib, ix->aos.as<float>(), ix->n, g, e->stream);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
   ix->built = true;
@@ -754,24 +925,31 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   if ((!centrals && n) || stride_bytes < 12 || (!counts && n))
     return fail(e, SJCS_ERR_INVALID, "null pointer or stride < 12");
   int rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // the sort workspace is shared with the chunk path
   if (rc != SJCS_OK) return rc;
   if (n == 0) return SJCS_OK;
   ShellState sh;
   HaloState hs;
-  DevBuf pts, dcounts;
+  DevBuf pts, dcounts, ucounts;
   rc = make_shells(e, sh, radiuses_sq, nrads);
+  if (rc == SJCS_OK && !(sh.view.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
+    rc = fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
   if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
   if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), n);
   const size_t nc = (size_t)n * (size_t)nrads;
   std::vector<uint64_t> host;
   if (rc == SJCS_OK) {
-    hipError_t h = ensure(dcounts, nc * sizeof(uint64_t));
+    hipError_t h = ensure(dcounts, nc * sizeof(uint32_t));
+    if (h == hipSuccess) h = ensure(ucounts, nc * sizeof(uint64_t));
     if (h == hipSuccess)
-      h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint64_t>(), false,
+      h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint32_t>(), false,
                       nullptr, lanes(e), seg_len(e), e->stream);
+    if (h == hipSuccess)
+      h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.row.as<uint32_t>(),
+                         ucounts.as<uint64_t>(), n, nrads, e->stream);
     if (h == hipSuccess) {
       host.resize(nc);
-      h = hipMemcpyAsync(host.data(), dcounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
+      h = hipMemcpyAsync(host.data(), ucounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
                          e->stream);
     }
     if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
@@ -787,6 +965,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   hs.row.release();
   pts.release();
   dcounts.release();
+  ucounts.release();
   return rc;
 }
 

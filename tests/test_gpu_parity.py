@@ -281,6 +281,63 @@ def test_chunk_accumulation_and_reset(eng):
     assert_counts_equal(eng.counts(), oracle_chunks([p[:10_000]], h, r2)[0], h)
 
 
+def test_pipelined_chunks(eng):
+    """Chunks alternate between host and device input and grow in size (the
+    two index slots, staging buffers and workspace reallocate mid-stream);
+    a reset, a new halo set and new shells between chunk streams."""
+    p, h = gen_uniform(23, 120_000, 3_000, face=0.2)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 16)
+    cuts = [0, 5_000, 5_001, 25_000, 60_000, 61_000, 120_000]
+    chunks = [p[a:b] for a, b in zip(cuts[:-1], cuts[1:])]
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    dev = [eng.device_array(c) if i % 2 else None for i, c in enumerate(chunks)]
+    try:
+        for c, d in zip(chunks, dev):
+            if d is None:
+                eng.add_particles(c)
+            else:
+                eng.add_particles_device(d, c.shape[0])
+        assert_counts_equal(eng.counts(), oracle_chunks(chunks, h, r2)[0], h)
+        eng.reset_counts()
+        for c, d in zip(chunks[::-1], dev[::-1]):
+            if d is None:
+                eng.add_particles(c)
+            else:
+                eng.add_particles_device(d, c.shape[0])
+        h2 = h[::2].copy()
+        _, r2b = t.oracle_thresholds(0.01, 8.0, 9)
+        eng.set_halos(h2)
+        eng.set_shells(r2b)
+        for c in chunks[:3]:
+            eng.add_particles(c)
+        assert_counts_equal(eng.counts(), oracle_chunks(chunks[:3], h2, r2b)[0], h2)
+    finally:
+        for d in dev:
+            if d is not None:
+                eng.device_free(d)
+
+
+@pytest.mark.parametrize("limit", [1, 45_000])
+def test_count_folds(eng, limit):
+    """Device counts run in u32 and fold into u64 totals before they could
+    overflow; a tiny fold limit folds after every chunk (or every other), and
+    read-out adds the u64 totals and the running u32 counts."""
+    p, h = gen_uniform(24, 90_000, 2_500, face=0.2)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 10)
+    chunks = [p[:20_000], p[20_000:40_000], p[40_000:41_000], p[41_000:]]
+    eng.set_option("fold_limit", limit)
+    try:
+        got = run_engine(eng, chunks, h, r2)
+        assert_counts_equal(got, oracle_chunks(chunks, h, r2)[0], h)
+        assert np.array_equal(eng.counts(), got)  # read-out repeats
+        eng.reset_counts()
+        eng.add_particles(chunks[0])
+        assert_counts_equal(eng.counts(), oracle_chunks(chunks[:1], h, r2)[0], h)
+    finally:
+        eng.set_option("fold_limit", 0)
+
+
 def test_degenerate_sizes(eng):
     _, r2 = t.oracle_thresholds(0.1, 5.0, 4)
     p, h = gen_uniform(17, 5_000, 50)
