@@ -51,7 +51,7 @@ CONFIGS = {
                np=100_000, nh=10_000, radius=(0.001, 5.0, 8), chunks=1),
     "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
                         "u64 counts accumulated on the device",
-               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3),
+               np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3, graph=True),
 }
 
 
@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--lanes-per-halo", type=int, default=0)
     ap.add_argument("--plain", action="store_true",
                     help="one halo per wavefront count kernel")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each chunk's sort + count from a captured HIP graph")
     ap.add_argument("--overlap", action="store_true",
                     help="sort the next chunk while this one is counted")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,6 +176,8 @@ def main():
         eng.set_option("kernel", 1)
     if args.overlap:
         eng.set_option("overlap", 1)
+    if args.graph or cfg.get("graph"):
+        eng.set_option("graph", 1)
     eng.set_shells(r2)
     eng.set_halos(halos)
     d_parts = [eng.device_array(p) for p in parts]
@@ -271,7 +275,9 @@ def main():
                    "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
                    "pairs_counted": counted, "parallelism": "single GPU"},
         "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4),
-                      "of": "the last chunk of a step" if cfg["chunks"] > 1 else "the chunk"},
+                      "of": ("the last chunk of a step" if cfg["chunks"] > 1 else "the chunk") +
+                            (" (graph replay: sort and count timed together as count)"
+                             if args.graph or cfg.get("graph") else "")},
         "count_kernel": {"pair_tests_per_s": w / (count_ms * 1e-3),
                          "valu_tflops_8_per_test": round(count_tflops, 3),
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),

@@ -195,6 +195,19 @@ struct sjcs_engine {
   int64_t x_split = 0;
   int64_t overlap = 0;
   int64_t fold_limit = 0;  // 0: fold before a u32 count could pass 2^32 - 1
+  int64_t graph = 0;       // 1: each piece's sort + count replayed from a graph
+
+  // Captured sort + count of a piece, per slot and store/accumulate; valid
+  // while input, size and `epoch` match.  `epoch` changes whenever a pointer
+  // or a value captured in a graph may have changed.
+  struct ChunkGraph {
+    hipGraphExec_t exec = nullptr;
+    const float* in = nullptr;
+    uint64_t n = 0;
+    uint64_t epoch = 0;
+  };
+  ChunkGraph graphs[2][2];
+  uint64_t epoch = 1;
 
   // last chunk: upload start/end, sort start/end, count start/end
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -255,6 +268,7 @@ int ensure_pinned(sjcs_engine* e, size_t bytes) {
 }
 
 int ensure_ws(sjcs_engine* e, uint64_t words) {
+  if (e->ws.bytes < words * 4) ++e->epoch;
   HIPCALL(e, ensure(e->ws, words * 4), "hipMalloc(workspace)");
   return SJCS_OK;
 }
@@ -399,16 +413,19 @@ int upload_points(sjcs_engine* e, DevBuf& dst, const float* pts, size_t stride, 
   return SJCS_OK;
 }
 
-// Builds the cell index of n particles at d_xyz (device, packed) into ib,
-// on stream st.  Buffers that must grow are reallocated only after all work
-// of the engine is done.
-int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
-                hipStream_t st) {
+bool sort_by_axis(const sjcs_engine* e, const Grid& g) {
+  return g.g <= kCsMaxCells && g.gx <= kCsMaxCellsX && e->index_sort != 1;
+}
+
+// Sizes ib and the workspace for n particles on grid g; *grew tells whether
+// anything was reallocated (after all work of the engine was done).
+int reserve_index(sjcs_engine* e, IndexBuffers& ib, uint64_t n, Grid g, bool* grew) {
+  *grew = false;
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles per chunk");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   // grids of at most kCsMaxCells cells per axis use the per-axis cell sort;
   // finer ones (or "index_sort" 1) the radix sort of packed cell keys
-  const bool by_axis = g.g <= kCsMaxCells && g.gx <= kCsMaxCellsX && e->index_sort != 1;
+  const bool by_axis = sort_by_axis(e, g);
   const uint64_t wsw = by_axis ? cell_sort_workspace_words(n)
                                : std::max(radix_workspace_words(n), scan_workspace_words(g.ncells));
   const size_t ceb = sizeof(uint32_t) * (size_t)g.ncells;
@@ -419,6 +436,8 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n
   if (grow) {
     int rc = quiesce(e);
     if (rc != SJCS_OK) return rc;
+    *grew = true;
+    ++e->epoch;
   }
   for (int i = 0; i < 2; ++i) {
     if (!by_axis) HIPCALL(e, ensure(ib.key[i], b), "hipMalloc(index)");
@@ -431,6 +450,14 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n
   if (rc != SJCS_OK) return rc;
   ib.grid = g;
   ib.n = n;
+  return SJCS_OK;
+}
+
+// Enqueues the index build of n particles at d_xyz into ib (sized by
+// reserve_index) on stream st; launches only.
+int enqueue_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
+                  hipStream_t st) {
+  const bool by_axis = sort_by_axis(e, g);
   if (by_axis) {
     float* xs[2] = {ib.x[0].as<float>(), ib.x[1].as<float>()};
     float* ys[2] = {ib.y[0].as<float>(), ib.y[1].as<float>()};
@@ -459,6 +486,17 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n
   return SJCS_OK;
 }
 
+// Builds the cell index of n particles at d_xyz (device, packed) into ib,
+// on stream st.  Buffers that must grow are reallocated only after all work
+// of the engine is done.
+int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
+                hipStream_t st) {
+  bool grew = false;
+  int rc = reserve_index(e, ib, n, g, &grew);
+  if (rc != SJCS_OK) return rc;
+  return enqueue_index(e, ib, d_xyz, n, g, st);
+}
+
 HaloView halo_view(const HaloState& h) {
   HaloView v;
   v.x = h.x.as<float>();
@@ -478,6 +516,7 @@ int prepare_counts(sjcs_engine* e) {
   if (e->counts.bytes < bytes) {
     int rc = quiesce(e);
     if (rc != SJCS_OK) return rc;
+    ++e->epoch;
   }
   HIPCALL(e, ensure(e->counts, bytes), "hipMalloc(counts)");
   return SJCS_OK;
@@ -512,6 +551,60 @@ int fold_counts(sjcs_engine* e) {
   return SJCS_OK;
 }
 
+int piece_done(sjcs_engine* e, uint64_t n);
+
+// Graph mode: sort and count of a piece on `stream` as one captured graph,
+// captured on first use and replayed while nothing it holds has changed.
+// The sort waits for the slot's upload; it follows the previous count in
+// stream order.
+int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
+  const int sl = e->slot;
+  bool grew = false;
+  int rc = reserve_index(e, e->ibs[sl], n, g, &grew);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, hipStreamWaitEvent(e->stream, e->uploaded[sl], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
+  sjcs_engine::ChunkGraph& cg = e->graphs[sl][e->counts_fresh ? 0 : 1];
+  if (!cg.exec || cg.in != d_xyz || cg.n != n || cg.epoch != e->epoch) {
+    if (cg.exec) (void)hipGraphExecDestroy(cg.exec);
+    cg.exec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIPCALL(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal),
+            "hipStreamBeginCapture");
+    int rq = enqueue_index(e, e->ibs[sl], d_xyz, n, g, e->stream);
+    hipError_t hc = hipSuccess;
+    if (rq == SJCS_OK)
+      hc = count_halos(e->ibs[sl].view(), halo_view(e->halos), e->shells.view,
+                       e->counts.as<uint32_t>(), !e->counts_fresh, nullptr, lanes(e), seg_len(e),
+                       e->stream);
+    const hipError_t he = hipStreamEndCapture(e->stream, &graph);
+    if (rq != SJCS_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rq;
+    }
+    if (hc != hipSuccess || he != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return hip_fail(e, hc != hipSuccess ? hc : he, "graph capture");
+    }
+    const hipError_t hi = hipGraphInstantiate(&cg.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (hi != hipSuccess) {
+      cg.exec = nullptr;
+      return hip_fail(e, hi, "hipGraphInstantiate");
+    }
+    cg.in = d_xyz;
+    cg.n = n;
+    cg.epoch = e->epoch;
+  }
+  HIPCALL(e, hipGraphLaunch(cg.exec, e->stream), "hipGraphLaunch");
+  HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->built[sl], e->stream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->counted[sl], e->stream), "hipEventRecord");
+  return piece_done(e, n);
+}
+
 // Sorts one piece into the current slot (bstream) and counts it (stream).
 int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   const int sl = e->slot;
@@ -522,6 +615,7 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
     if (rc != SJCS_OK) return rc;
   }
   const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli, e->x_split);
+  if (e->graph) return graph_piece(e, d_xyz, n, g);
   HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
   if (!e->overlap && e->last >= 0)
     HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[e->last], 0), "hipStreamWaitEvent");
@@ -538,10 +632,16 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
           "count kernel");
   HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->counted[sl], e->stream), "hipEventRecord");
+  return piece_done(e, n);
+}
+
+// Bookkeeping after a piece was enqueued in slot e->slot.
+int piece_done(sjcs_engine* e, uint64_t n) {
+  const int sl = e->slot;
   e->last = sl;
   e->slot = sl ^ 1;
   e->counts_fresh = false;
-  e->pending += img * n;
+  e->pending += images_max(e) * n;
   e->timed = true;
   return SJCS_OK;
 }
@@ -624,6 +724,8 @@ void sjcs_engine_destroy(sjcs_engine* e) {
   e->counts64.release();
   e->ucounts.release();
   for (int i = 0; i < 2; ++i) {
+    for (int j = 0; j < 2; ++j)
+      if (e->graphs[i][j].exec) (void)hipGraphExecDestroy(e->graphs[i][j].exec);
     e->ibs[i].release();
     e->cin[i].release();
     if (e->cpin[i]) (void)hipHostFree(e->cpin[i]);
@@ -651,6 +753,7 @@ int sjcs_set_shells(sjcs_engine* e, const float* radiuses_sq, int nrads) {
   int rc = use_device(e);
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the shells
   if (rc != SJCS_OK) return rc;
+  ++e->epoch;
   rc = make_shells(e, e->shells, radiuses_sq, nrads);
   if (rc != SJCS_OK) return rc;
   e->have_shells = true;
@@ -664,6 +767,7 @@ int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uin
   int rc = use_device(e);
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
+  ++e->epoch;
   rc = upload_points(e, e->in, points, stride_bytes, nhalos);
   if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos);
   if (rc != SJCS_OK) return rc;
@@ -677,6 +781,7 @@ int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
   int rc = use_device(e);
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
+  ++e->epoch;
   rc = sort_halos(e, e->halos, d_xyz, nhalos);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
@@ -704,9 +809,12 @@ int sjcs_add_particles(sjcs_engine* e, const float* xyz, uint64_t n) {
     HIPCALL(e, hipHostMalloc(&e->cpin[sl], bytes, hipHostMallocDefault), "hipHostMalloc");
     e->cpin_bytes[sl] = bytes;
   }
-  if (e->cin[sl].bytes < bytes) HIPCALL(e, hipEventSynchronize(e->built[sl]), "hipEventSynchronize");
+  if (e->cin[sl].bytes < bytes) HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
   HIPCALL(e, ensure(e->cin[sl], bytes), "hipMalloc(points)");
   if (n) memcpy(e->cpin[sl], xyz, 12 * (size_t)n);
+  // the staging is rewritten once the sort that read it (graph mode: the
+  // slot's last graph, on the count stream) is done
+  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
   if (n)
     HIPCALL(e, hipMemcpyAsync(e->cin[sl].p, e->cpin[sl], 12 * (size_t)n, hipMemcpyHostToDevice,
@@ -776,6 +884,17 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
 
 int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
   if (!e || !key) return SJCS_ERR_INVALID;
+  // options change which stream sorts run on and what graphs hold: let the
+  // work in flight finish first
+  int rq = use_device(e);
+  if (rq == SJCS_OK) rq = quiesce(e);
+  if (rq != SJCS_OK) return rq;
+  ++e->epoch;
+  if (strcmp(key, "graph") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "graph: 0 or 1");
+    e->graph = value;
+    return SJCS_OK;
+  }
   if (strcmp(key, "cell_size_milli") == 0) {
     if (value < 0 || (value > 0 && value < 977))  // at most 1024 cells per axis
       return fail(e, SJCS_ERR_INVALID, "cell_size_milli: 0 or >= 977");

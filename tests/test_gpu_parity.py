@@ -338,6 +338,42 @@ def test_count_folds(eng, limit):
         eng.set_option("fold_limit", 0)
 
 
+def test_graph_chunk_loop(eng):
+    """Graph mode: each chunk's sort + count is captured once and replayed.
+    Every replay (device and host input, store and accumulate graphs) equals
+    the eager run, with eager kernels and synchronisations between replays;
+    a new halo set recaptures."""
+    p, h = gen_uniform(25, 70_000, 2_500, face=0.2)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 12)
+    chunks = [p[:30_000], p[30_000:]]
+    eager = run_engine(eng, chunks, h, r2)
+    assert_counts_equal(eager, oracle_chunks(chunks, h, r2)[0], h)
+    w_eager = eng.last_chunk_pair_tests()
+    d = [eng.device_array(c) for c in chunks]
+    eng.set_option("graph", 1)
+    try:
+        for rep in range(4):
+            eng.reset_counts()
+            for c, dd in zip(chunks, d):
+                eng.add_particles_device(dd, c.shape[0])
+            assert np.array_equal(eng.counts(), eager), f"device replay {rep}"
+            assert eng.last_chunk_pair_tests() == w_eager
+            eng.reset_counts()
+            for c in chunks:
+                eng.add_particles(c)
+            assert np.array_equal(eng.counts(), eager), f"host replay {rep}"
+        h2 = h[::3].copy()
+        eng.set_halos(h2)
+        eng.reset_counts()
+        for c, dd in zip(chunks, d):
+            eng.add_particles_device(dd, c.shape[0])
+        assert_counts_equal(eng.counts(), oracle_chunks(chunks, h2, r2)[0], h2)
+    finally:
+        eng.set_option("graph", 0)
+        for dd in d:
+            eng.device_free(dd)
+
+
 def test_degenerate_sizes(eng):
     _, r2 = t.oracle_thresholds(0.1, 5.0, 4)
     p, h = gen_uniform(17, 5_000, 50)
