@@ -39,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import spatialjoincountovershells_amd as sj  # noqa: E402
+from spatialjoincountovershells_amd import synth  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "astr_rcount")
 HBM_PEAK_GBS = 8000.0
@@ -49,6 +50,11 @@ CONFIGS = {
                np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=1),
     "c1": dict(workload="uniform 1e5 particles x 1e4 halos, --radius 0.001:5:8, one chunk",
                np=100_000, nh=10_000, radius=(0.001, 5.0, 8), chunks=1),
+    "c4": dict(workload="clustered 2e8 particles (70% in 200k Gaussian clumps, sigma 0.3-2, "
+                        "power-law sizes) x 5e6 halos (80% at clump centres), "
+                        "--radius 0.01:20:64, one chunk",
+               np=200_000_000, nh=5_000_000, radius=(0.01, 20.0, 64), chunks=1,
+               clustered=True),
     "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
                         "u64 counts accumulated on the device",
                np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3, graph=True),
@@ -59,20 +65,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def uniform_points(rng, n):
-    pts = rng.random((n, 3), dtype=np.float32) * np.float32(1000.0)
-    # float32 rounding of u*1000 can give exactly 1000.0: keep [0, 1000)
-    np.minimum(pts, np.nextafter(np.float32(1000.0), np.float32(0)), out=pts)
-    return pts
-
-
 def make_inputs(cfg, seed):
     """Particle files (one array each, distinct seeds) and the halos."""
     rng = np.random.default_rng(seed)
-    parts = [uniform_points(rng, cfg["np"])]
-    halos = uniform_points(rng, cfg["nh"])
+    if cfg.get("clustered"):
+        parts, halos = synth.clustered(rng, cfg["np"], cfg["nh"])
+        return [parts], halos
+    parts = [synth.uniform_points(rng, cfg["np"])]
+    halos = synth.uniform_points(rng, cfg["nh"])
     for c in range(1, cfg["chunks"]):
-        parts.append(uniform_points(np.random.default_rng(seed + 1000 * c), cfg["np"]))
+        parts.append(synth.uniform_points(np.random.default_rng(seed + 1000 * c), cfg["np"]))
     return parts, halos
 
 
@@ -89,7 +91,9 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
             pfs.append(os.path.join(tmp, f"part{c}.dat"))
             p.tofile(pfs[-1])
         hf = os.path.join(tmp, "halo.list")
-        hs = halos[:nsample]
+        # a random sample (the clustered set lists its clump-centre halos first)
+        pick = np.sort(np.random.default_rng(7).choice(halos.shape[0], nsample, replace=False))
+        hs = halos[pick]
         with open(hf, "w") as f:
             for i in range(16):
                 f.write(f"# dummy header {i}\n")
@@ -120,7 +124,7 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
         return {"value": w / search_s, "unit": "pair-tests/s", "cores": threads,
                 "kind": "reference",
                 "sample": f"astr_rcount --thread {threads}: all {npart} particles "
-                          f"({len(parts)} file(s)) x {nsample} halos (first {nsample} of the "
+                          f"({len(parts)} file(s)) x {nsample} halos (random sample of the "
                           f"same set), {mn}:{mx}:{k}; W={w} in {search_s:.3f} s of "
                           f"'searching time' (process wall {wall:.1f} s)"}
     finally:
@@ -246,10 +250,11 @@ def main():
                 "bytes_per_launch": build_bytes}
 
     cpu = None
-    if not args.no_cpu_baseline and args.config in ("c2", "c3"):
+    if not args.no_cpu_baseline and args.config in ("c2", "c3", "c4"):
         log("[bench] cpu baseline (reference astr_rcount on a halo sample)")
+        nsample = args.cpu_halos if args.config != "c4" else min(args.cpu_halos, nh // 100)
         cpu = cpu_baseline(parts, halos, cfg["radius"], eng, d_parts,
-                           min(args.cpu_halos, nh), args.cpu_threads)
+                           min(nsample, nh), args.cpu_threads)
 
     for d in d_parts:
         eng.device_free(d)

@@ -111,6 +111,9 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *                       automatic (16 up to 256 shells)
  *   "segment"           particles of a cell row one lane walks: 4 or 8,
  *                       0 = automatic
+ *   "tiles"             1 (default) = halos sharing a Morton cell (runs of
+ *                       at least 4) are counted as tiles that share their
+ *                       candidate particles; 0 = one halo at a time
  *   "graph"             1 = each chunk's sort + count is captured once as
  *                       a HIP graph and replayed (timings then give the
  *                       pair as the count phase, sort phase 0)

@@ -17,21 +17,12 @@
 // test, FP32 distance and first-match shell, and lands in the group's LDS
 // histogram, flushed to the u64 counts when the halo is done.
 #include "count.h"
+#include "pairs.h"
 
 #pragma clang fp contract(off)
 
 namespace sjcs {
 namespace {
-
-constexpr int kShellScan = 0;     // any thresholds: first match by linear scan
-constexpr int kShellTable = 1;    // sorted: table lower bound + upward scan
-constexpr int kShellOneStep = 2;  // sorted, one threshold per bucket at most
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // DPP row_shr:n within each row of 16 lanes; lanes whose source falls before
 // the row start read 0.
@@ -103,56 +94,6 @@ __device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
 #pragma unroll
   for (int off = GROUP; off < 64; off <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
   return v;
-}
-
-// First k with d < r2[k] (src/objects/ArraySTR/MultiRangeCount.c:107-113),
-// as the byte offset 4k of its histogram bin, for a d < r2[nrads-1] (the
-// caller's gate); the scan mode takes any d and returns -1 when no shell
-// matches.
-template <int kShell>
-__device__ __forceinline__ int shell_off(float d, const float* r2, const ShellEntry* lut,
-                                         const Shells& sh) {
-  if constexpr (kShell == kShellScan) {
-    for (int k = 0; k < sh.nrads; ++k)
-      if (d < r2[k]) return 4 * k;
-    return -1;
-  } else {
-    // d < r2max bounds the bucket above; bits below the base clamp to 0
-    const int b = max((int)(__float_as_uint(d) - sh.lut_base) >> sh.lut_shift, 0);
-    const ShellEntry e = lut[b];
-    int off = (int)e.off + (d < e.t ? 0 : 4);
-    if constexpr (kShell == kShellTable)
-      while (!(d < r2[off >> 2])) off += 4;  // ends: d < r2[nrads-1], r2 ascending
-    return off;
-  }
-}
-
-// FP32 distance exactly as DPPS 0x77 rounds it (include/objects/RTree/Distance.h:11-13).
-__device__ __forceinline__ float dpps_distance(float px, float py, float pz, float cx,
-                                               float cy, float cz) {
-  const float dx = px - cx;
-  const float dy = py - cy;
-  const float dz = pz - cz;
-  const float xx = dx * dx;
-  const float yy = dy * dy;
-  const float zz = dz * dz;
-  const float sxy = xx + yy;
-  return sxy + zz;
-}
-
-// gfx9 buffer descriptor word 3 (raw buffer, 32-bit data); out-of-range
-// loads return 0, so lanes past the end need no branch.
-constexpr int kBufferFlags = 0x00020000;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float* p, uint64_t n) {
-  const uint64_t bytes = n * 4;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0,
-                                           (int)(bytes < 0xffffffffull ? bytes : 0xffffffffull),
-                                           kBufferFlags);
-}
-
-__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
 
 template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox>
@@ -229,8 +170,10 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     if (kLds && !kInbox) {
       for (int k = gl; k < K; k += GROUP) bins[k] = 0u;
     }
-    // A NaN centre makes every distance NaN: nothing is ever counted.
-    const bool live = active && (cx == cx) && (cy == cy) && (cz == cz);
+    // A NaN centre makes every distance NaN: nothing is ever counted.  Its
+    // box still passes every coordinate on the NaN axes, and the in-box
+    // measurement counts those pair tests as the reference makes them.
+    const bool live = active && (kInbox || ((cx == cx) && (cy == cy) && (cz == cz)));
     // Periodic images (src/objects/ArraySTR/MultiRangeCount.c:31-71,
     // src/objects/RTree/Mbr.c:284-313): an axis whose base box leaves
     // [0, 1000] is shifted by +1000 (low side wins) or -1000.
@@ -264,17 +207,9 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       const float lx = sx - rbox, hx = sx + rbox;
       const float ly = sy - rbox, hy = sy + rbox;
       const float lz = sz - rbox, hz = sz + rbox;
-      // Below `safe` a distance proves the box test: with m = max(|sx|, |sy|,
-      // |sz|) and A = rbox - (m + rbox) * 2^-21, dist < A^2 (1 - 2^-19) gives
-      // |p - s| < A (1 + 2^-24) on every axis (each squared term is at most
-      // the FP32 sum), which keeps p inside [fl(s - rbox), fl(s + rbox)] with
-      // room for those two roundings.  A non-finite centre gets no bound.
+      // below `safe` a distance proves the box test
       float safe = 0.0f;
-      if constexpr (kSphere) {
-        const float m = fmaxf(fmaxf(fabsf(sx), fabsf(sy)), fabsf(sz));
-        const float a = rbox - (m + rbox) * 0x1p-21f;
-        safe = a > 0.0f ? (a * a) * (1.0f - 0x1p-19f) : 0.0f;
-      }
+      if constexpr (kSphere) safe = safe_dist2(sx, sy, sz, rbox);
       int x0, x1, y0, y1, z0, z1;
       if constexpr (kSphere) {
         x0 = cell_of(lx, invx, gxm1), x1 = cell_of(hx, invx, gxm1);

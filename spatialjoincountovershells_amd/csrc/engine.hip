@@ -17,6 +17,7 @@
 #include "common.h"
 #include "count.h"
 #include "primitives.h"
+#include "tile.h"
 
 using namespace sjcs;
 
@@ -57,9 +58,32 @@ struct ShellState {
   Shells view{};
 };
 
+// Halos in Morton order, with runs of at least kTileMin halos sharing a
+// Morton cell moved to the front ([0, n_tiled)); `runs` lists them.  Tiles
+// for a shell count are cut from the runs (tile_cap halos at most).
+constexpr uint32_t kTileMin = 4;
+
 struct HaloState {
   uint64_t n = 0;
   DevBuf x, y, z, row;
+  uint64_t n_tiled = 0;
+  std::vector<uint32_t> run_start, run_len;
+  DevBuf tstart, tlen;  // tiles for tile_cap == tcap
+  uint32_t ntiles = 0;
+  int tcap = 0;
+  void release() {
+    x.release();
+    y.release();
+    z.release();
+    row.release();
+    tstart.release();
+    tlen.release();
+    n_tiled = 0;
+    run_start.clear();
+    run_len.clear();
+    ntiles = 0;
+    tcap = 0;
+  }
 };
 
 // A built particle index: sorted SoA coordinates and the cell table, plus
@@ -196,6 +220,7 @@ struct sjcs_engine {
   int64_t overlap = 0;
   int64_t fold_limit = 0;  // 0: fold before a u32 count could pass 2^32 - 1
   int64_t graph = 0;       // 1: each piece's sort + count replayed from a graph
+  int64_t tiles = 1;       // 1: halo tiles share their candidates (ascending shells)
 
   // Captured sort + count of a piece, per slot and store/accumulate; valid
   // while input, size and `epoch` match.  `epoch` changes whenever a pointer
@@ -343,9 +368,8 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
 }
 
 // Halo centres from a device AoS (packed xyz) -> Morton-sorted SoA + rows.
-int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n) {
+int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n, bool tiles) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 halos");
-  hs.n = n;
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   DevBuf key[2], x[2], y[2], z[2], row[2];
   int rc = SJCS_OK;
@@ -371,11 +395,62 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n) {
   }
   if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
   if (h != hipSuccess) rc = hip_fail(e, h, "halo sort");
+  // runs of >= kTileMin halos with one Morton key go first, in order
+  std::vector<uint32_t> rs, rl;
+  uint64_t ntiled = 0;
+  if (rc == SJCS_OK && tiles && n >= kTileMin) {
+    std::vector<uint32_t> keys(n);
+    h = hipMemcpy(keys.data(), key[sel].p, 4 * n, hipMemcpyDeviceToHost);
+    if (h != hipSuccess) rc = hip_fail(e, h, "halo keys");
+    for (uint64_t i = 0; rc == SJCS_OK && i < n;) {
+      uint64_t j = i + 1;
+      while (j < n && keys[j] == keys[i]) ++j;
+      if (j - i >= kTileMin) {
+        rs.push_back((uint32_t)i);
+        rl.push_back((uint32_t)(j - i));
+        ntiled += j - i;
+      }
+      i = j;
+    }
+    if (rc == SJCS_OK && ntiled) {
+      std::vector<uint32_t> perm;
+      perm.reserve(n);
+      std::vector<uint32_t> nstart(rs.size());
+      for (size_t r = 0; r < rs.size(); ++r) {
+        nstart[r] = (uint32_t)perm.size();
+        for (uint32_t k = 0; k < rl[r]; ++k) perm.push_back(rs[r] + k);
+      }
+      size_t r = 0;
+      for (uint64_t i = 0; i < n; ++i) {
+        if (r < rs.size() && i == rs[r]) {
+          i += rl[r] - 1;
+          ++r;
+          continue;
+        }
+        perm.push_back((uint32_t)i);
+      }
+      rs.swap(nstart);
+      DevBuf dperm;
+      h = ensure(dperm, 4 * n);
+      if (h == hipSuccess) h = hipMemcpy(dperm.p, perm.data(), 4 * n, hipMemcpyHostToDevice);
+      const int o = sel ^ 1;
+      if (h == hipSuccess)
+        h = permute_halos(x[sel].as<float>(), y[sel].as<float>(), z[sel].as<float>(),
+                          row[sel].as<uint32_t>(), x[o].as<float>(), y[o].as<float>(),
+                          z[o].as<float>(), row[o].as<uint32_t>(), dperm.as<uint32_t>(), n,
+                          e->stream);
+      if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+      dperm.release();
+      if (h != hipSuccess) rc = hip_fail(e, h, "halo tiling");
+      sel = o;
+    }
+  }
   if (rc == SJCS_OK) {
-    hs.x.release();
-    hs.y.release();
-    hs.z.release();
-    hs.row.release();
+    hs.release();
+    hs.n = n;
+    hs.n_tiled = ntiled;
+    hs.run_start.swap(rs);
+    hs.run_len.swap(rl);
     hs.x = x[sel];
     hs.y = y[sel];
     hs.z = z[sel];
@@ -551,6 +626,63 @@ int fold_counts(sjcs_engine* e) {
   return SJCS_OK;
 }
 
+// Tiles of the current halos for the current shells (host runs cut into
+// pieces of at most tile_cap halos, uploaded once); false when the count
+// runs without tiles.
+int ensure_tiles(sjcs_engine* e, bool* use) {
+  *use = false;
+  HaloState& hs = e->halos;
+  const int cap = tile_cap(e->shells.nrads);
+  if (!e->tiles || !e->shells.view.sorted || hs.n_tiled == 0 || cap < 2) return SJCS_OK;
+  *use = true;
+  if (hs.tcap == cap) return SJCS_OK;
+  std::vector<uint32_t> ts, tl;
+  for (size_t r = 0; r < hs.run_start.size(); ++r) {
+    const uint32_t len = hs.run_len[r];
+    const uint32_t parts = (len + cap - 1) / cap;
+    for (uint32_t p = 0, at = 0; p < parts; ++p) {
+      const uint32_t m = len / parts + (p < len % parts ? 1 : 0);
+      ts.push_back(hs.run_start[r] + at);
+      tl.push_back(m);
+      at += m;
+    }
+  }
+  int rc = quiesce(e);  // the tile table is read by counts in flight
+  if (rc != SJCS_OK) return rc;
+  ++e->epoch;
+  HIPCALL(e, ensure(hs.tstart, 4 * ts.size()), "hipMalloc(tiles)");
+  HIPCALL(e, ensure(hs.tlen, 4 * tl.size()), "hipMalloc(tiles)");
+  HIPCALL(e, hipMemcpy(hs.tstart.p, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  HIPCALL(e, hipMemcpy(hs.tlen.p, tl.data(), 4 * tl.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  hs.ntiles = (uint32_t)ts.size();
+  hs.tcap = cap;
+  return SJCS_OK;
+}
+
+// Launches the count of the current halos against index `ix` on stream st:
+// tiles (when `use_tiles`) and then the remaining halos one by one.
+hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint32_t* counts,
+                       bool accumulate, unsigned long long* inbox, hipStream_t st) {
+  HaloView hv = halo_view(e->halos);
+  const Shells& sh = e->shells.view;
+  if (!use_tiles)
+    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e), seg_len(e), st);
+  TileView tv;
+  tv.start = e->halos.tstart.as<uint32_t>();
+  tv.len = e->halos.tlen.as<uint32_t>();
+  tv.n = e->halos.ntiles;
+  hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox, st);
+  if (h != hipSuccess) return h;
+  const uint64_t nt = e->halos.n_tiled;
+  hv.x += nt;
+  hv.y += nt;
+  hv.z += nt;
+  hv.orig += nt;
+  hv.n -= nt;
+  return count_halos(ix, hv, sh, counts ? counts + nt * (uint64_t)sh.nrads : nullptr, accumulate,
+                     inbox, lanes(e), seg_len(e), st);
+}
+
 int piece_done(sjcs_engine* e, uint64_t n);
 
 // Graph mode: sort and count of a piece on `stream` as one captured graph,
@@ -559,8 +691,9 @@ int piece_done(sjcs_engine* e, uint64_t n);
 // stream order.
 int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
   const int sl = e->slot;
-  bool grew = false;
+  bool grew = false, use_tiles = false;
   int rc = reserve_index(e, e->ibs[sl], n, g, &grew);
+  if (rc == SJCS_OK) rc = ensure_tiles(e, &use_tiles);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipStreamWaitEvent(e->stream, e->uploaded[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
@@ -576,9 +709,8 @@ int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
     int rq = enqueue_index(e, e->ibs[sl], d_xyz, n, g, e->stream);
     hipError_t hc = hipSuccess;
     if (rq == SJCS_OK)
-      hc = count_halos(e->ibs[sl].view(), halo_view(e->halos), e->shells.view,
-                       e->counts.as<uint32_t>(), !e->counts_fresh, nullptr, lanes(e), seg_len(e),
-                       e->stream);
+      hc = count_chunk(e, e->ibs[sl].view(), use_tiles, e->counts.as<uint32_t>(), !e->counts_fresh,
+                       nullptr, e->stream);
     const hipError_t he = hipStreamEndCapture(e->stream, &graph);
     if (rq != SJCS_OK) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -616,6 +748,9 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   }
   const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli, e->x_split);
   if (e->graph) return graph_piece(e, d_xyz, n, g);
+  bool use_tiles = false;
+  int rt = ensure_tiles(e, &use_tiles);
+  if (rt != SJCS_OK) return rt;
   HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
   if (!e->overlap && e->last >= 0)
     HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[e->last], 0), "hipStreamWaitEvent");
@@ -626,9 +761,8 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   HIPCALL(e, hipEventRecord(e->built[sl], e->bstream), "hipEventRecord");
   HIPCALL(e, hipStreamWaitEvent(e->stream, e->built[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
-  HIPCALL(e, count_halos(e->ibs[sl].view(), halo_view(e->halos), e->shells.view,
-                         e->counts.as<uint32_t>(), !e->counts_fresh, nullptr, lanes(e),
-                         seg_len(e), e->stream),
+  HIPCALL(e, count_chunk(e, e->ibs[sl].view(), use_tiles, e->counts.as<uint32_t>(),
+                         !e->counts_fresh, nullptr, e->stream),
           "count kernel");
   HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->counted[sl], e->stream), "hipEventRecord");
@@ -716,10 +850,7 @@ void sjcs_engine_destroy(sjcs_engine* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   e->shells.d_r2.release();
   e->shells.d_lut.release();
-  e->halos.x.release();
-  e->halos.y.release();
-  e->halos.z.release();
-  e->halos.row.release();
+  e->halos.release();
   e->counts.release();
   e->counts64.release();
   e->ucounts.release();
@@ -769,7 +900,7 @@ int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uin
   if (rc != SJCS_OK) return rc;
   ++e->epoch;
   rc = upload_points(e, e->in, points, stride_bytes, nhalos);
-  if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos);
+  if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -782,7 +913,7 @@ int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   ++e->epoch;
-  rc = sort_halos(e, e->halos, d_xyz, nhalos);
+  rc = sort_halos(e, e->halos, d_xyz, nhalos, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -873,8 +1004,11 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
   *pair_tests = 0;
   if (!e->timed) return SJCS_OK;
   HIPCALL(e, fill_u64(e->inbox.as<uint64_t>(), 0ull, 1, e->stream), "fill");
-  HIPCALL(e, count_halos(e->ibs[e->last].view(), halo_view(e->halos), e->shells.view, nullptr, true,
-                         e->inbox.as<unsigned long long>(), lanes(e), seg_len(e), e->stream),
+  bool use_tiles = false;
+  rc = ensure_tiles(e, &use_tiles);
+  if (rc != SJCS_OK) return rc;
+  HIPCALL(e, count_chunk(e, e->ibs[e->last].view(), use_tiles, nullptr, true,
+                         e->inbox.as<unsigned long long>(), e->stream),
           "pair-test count");
   HIPCALL(e, hipMemcpyAsync(pair_tests, e->inbox.p, 8, hipMemcpyDeviceToHost, e->stream),
           "hipMemcpy");
@@ -890,6 +1024,11 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
   if (rq == SJCS_OK) rq = quiesce(e);
   if (rq != SJCS_OK) return rq;
   ++e->epoch;
+  if (strcmp(key, "tiles") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "tiles: 0 or 1");
+    e->tiles = value;
+    return SJCS_OK;
+  }
   if (strcmp(key, "graph") == 0) {
     if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "graph: 0 or 1");
     e->graph = value;
@@ -1054,7 +1193,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   if (rc == SJCS_OK && !(sh.view.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
     rc = fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
   if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
-  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), n);
+  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), n, false);
   const size_t nc = (size_t)n * (size_t)nrads;
   std::vector<uint64_t> host;
   if (rc == SJCS_OK) {
@@ -1078,10 +1217,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
     for (size_t i = 0; i < nc; ++i) counts[i] += host[i];
   sh.d_r2.release();
   sh.d_lut.release();
-  hs.x.release();
-  hs.y.release();
-  hs.z.release();
-  hs.row.release();
+  hs.release();
   pts.release();
   dcounts.release();
   ucounts.release();

@@ -183,6 +183,8 @@ def test_special_coordinates(eng):
     want, w = oracle_chunks([p], h, r2)
     assert_counts_equal(got, want, h)
     assert got[0].sum() == 0 and got[8].sum() == 0
+    # a NaN centre's box passes every particle on its NaN axis: W counts them
+    assert eng.last_chunk_pair_tests() == w
 
 
 @pytest.mark.parametrize("index_sort", [0, 1])
@@ -372,6 +374,45 @@ def test_graph_chunk_loop(eng):
         eng.set_option("graph", 0)
         for dd in d:
             eng.device_free(dd)
+
+
+def tiled_halo_set(rng):
+    """Halos that form tiles (runs of >= 4 sharing a Morton cell of the
+    1024^3 grid): exact duplicates at a box corner (periodic images), a run of
+    70 spread inside one cell (split at the tile cap), NaN centres (Morton
+    cell 0) next to real halos of cell 0, and uniform singles."""
+    m = np.float32(1000.0 / 1024.0)
+    corner = np.repeat(np.array([[0.3, 999.8, 0.2]], np.float32), 40, axis=0)
+    cell = ((np.array([300, 611, 47], np.float32) + np.float32(0.1)) * m +
+            rng.uniform(0, 0.8, (70, 3)).astype(np.float32) * m)
+    nanrun = np.array([[np.nan, 1, 1]] * 5 + [[0.1, 0.2, 0.3]] * 3, np.float32)
+    singles = rng.random((500, 3), dtype=np.float32) * np.float32(1000.0)
+    return np.concatenate([singles[:200], corner, cell, nanrun, singles[200:]]).astype(np.float32)
+
+
+@pytest.mark.parametrize("spec", ["0.001:5:40", "0.01:20:64", "0.5:100:16", "1:600:5"])
+def test_halo_tiles(eng, spec):
+    """Tiles (shared candidates, one wavefront per tile) against the oracle,
+    and equal to the one-halo-at-a-time count, W included."""
+    rng = np.random.default_rng(26)
+    p = rng.random((60_000, 3), dtype=np.float32) * np.float32(1000.0)
+    clump = (np.array([1.0, 999.0, 2.0], np.float32) +
+             rng.normal(0, 1.5, (20_000, 3))).astype(np.float32) % np.float32(1000.0)
+    p = np.concatenate([p, clump]).astype(np.float32)
+    h = tiled_halo_set(rng)
+    mn, mx, k = t.parse_radius(spec)
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    got = run_engine(eng, [p], h, r2)
+    w_tiles = eng.last_chunk_pair_tests()
+    want, w = oracle_chunks([p], h, r2)
+    assert_counts_equal(got, want, h)
+    assert w_tiles == w
+    eng.set_option("tiles", 0)
+    try:
+        assert np.array_equal(run_engine(eng, [p], h, r2), got)
+        assert eng.last_chunk_pair_tests() == w
+    finally:
+        eng.set_option("tiles", 1)
 
 
 def test_degenerate_sizes(eng):
