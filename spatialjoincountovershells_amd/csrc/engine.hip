@@ -69,6 +69,7 @@ struct HaloState {
   uint64_t n_tiled = 0;
   std::vector<uint32_t> run_start, run_len;
   DevBuf tstart, tlen;  // tiles for tile_cap == tcap
+  DevBuf twork;         // count_tiles workspace
   uint32_t ntiles = 0;
   int tcap = 0;
   void release() {
@@ -78,6 +79,7 @@ struct HaloState {
     row.release();
     tstart.release();
     tlen.release();
+    twork.release();
     n_tiled = 0;
     run_start.clear();
     run_len.clear();
@@ -652,6 +654,7 @@ int ensure_tiles(sjcs_engine* e, bool* use) {
   ++e->epoch;
   HIPCALL(e, ensure(hs.tstart, 4 * ts.size()), "hipMalloc(tiles)");
   HIPCALL(e, ensure(hs.tlen, 4 * tl.size()), "hipMalloc(tiles)");
+  HIPCALL(e, ensure(hs.twork, 4 * tile_work_words((uint32_t)ts.size())), "hipMalloc(tiles)");
   HIPCALL(e, hipMemcpy(hs.tstart.p, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice), "hipMemcpy");
   HIPCALL(e, hipMemcpy(hs.tlen.p, tl.data(), 4 * tl.size(), hipMemcpyHostToDevice), "hipMemcpy");
   hs.ntiles = (uint32_t)ts.size();
@@ -671,7 +674,8 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint
   tv.start = e->halos.tstart.as<uint32_t>();
   tv.len = e->halos.tlen.as<uint32_t>();
   tv.n = e->halos.ntiles;
-  hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox, st);
+  hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox,
+                             e->halos.twork.as<uint32_t>(), st);
   if (h != hipSuccess) return h;
   const uint64_t nt = e->halos.n_tiled;
   hv.x += nt;

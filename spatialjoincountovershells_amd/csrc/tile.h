@@ -29,12 +29,16 @@ constexpr int kTileMaxHalos = 64;
 constexpr int kTileBinWords = 2048;
 int tile_cap(int nrads);
 
+// Workspace (u32 words) count_tiles needs for ntiles tiles.
+uint64_t tile_work_words(uint32_t ntiles);
+
 // counts[i*nrads + k] (= or +=) of every halo i of every tile, for ascending
 // thresholds only (Shells::sorted).  With `inbox` not null, nothing is
-// counted and inbox receives += the in-box pair tests.  No allocation.
+// counted and inbox receives += the in-box pair tests.  `work` holds
+// tile_work_words(tv.n) words.  No allocation.
 hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh,
                        const TileView& tv, uint32_t* counts, bool accumulate,
-                       unsigned long long* inbox, hipStream_t s);
+                       unsigned long long* inbox, uint32_t* work, hipStream_t s);
 
 // dst[i] = src[perm[i]] for the four halo arrays (x, y, z, row).
 hipError_t permute_halos(const float* sx, const float* sy, const float* sz, const uint32_t* srow,
