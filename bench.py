@@ -54,10 +54,18 @@ CONFIGS = {
                         "power-law sizes) x 5e6 halos (80% at clump centres), "
                         "--radius 0.01:20:64, one chunk",
                np=200_000_000, nh=5_000_000, radius=(0.01, 20.0, 64), chunks=1,
-               clustered=True),
+               clustered=True, autotune=True),
     "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
                         "u64 counts accumulated on the device",
                np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3, graph=True),
+}
+
+
+METRICS = {
+    "c1": "shell pair-tests/s (10k halos x 100k particles x 8 shells, 1 GPU)",
+    "c2": "shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)",
+    "c3": "shell pair-tests/s (10M halos x 3 x 100M particles x 40 shells, 1 GPU)",
+    "c4": "shell pair-tests/s (5M clustered halos x 200M clustered particles x 64 shells, 1 GPU)",
 }
 
 
@@ -141,7 +149,9 @@ def main():
     ap.add_argument("--cell-size-milli", type=int, default=0)
     ap.add_argument("--lanes-per-halo", type=int, default=0)
     ap.add_argument("--plain", action="store_true",
-                    help="one halo per wavefront count kernel")
+                    help="the plain count: one halo per wavefront, no halo tiles, no autotuning")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the engine's autotuner on the first particle file before timing")
     ap.add_argument("--graph", action="store_true",
                     help="replay each chunk's sort + count from a captured HIP graph")
     ap.add_argument("--overlap", action="store_true",
@@ -178,6 +188,7 @@ def main():
         eng.set_option("lanes_per_halo", args.lanes_per_halo)
     if args.plain:
         eng.set_option("kernel", 1)
+        eng.set_option("tiles", 0)
     if args.overlap:
         eng.set_option("overlap", 1)
     if args.graph or cfg.get("graph"):
@@ -191,6 +202,15 @@ def main():
         eng.reset_counts()
         for d in d_parts:
             eng.add_particles_device(d, n)
+
+    tuned = None
+    if args.autotune or (cfg.get("autotune") and not args.plain):
+        t_at = time.perf_counter()
+        best = eng.autotune(d_parts[0], n)
+        tuned = {"wall_s": round(time.perf_counter() - t_at, 3), "best_ms": round(best, 3),
+                 "chosen": {k: eng.get_option(k) for k in
+                            ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles")}}
+        log(f"[bench] autotune: {tuned}")
 
     # exact W of one step: an untimed pair-test pass per chunk
     w = 0
@@ -261,9 +281,7 @@ def main():
     eng.close()
     value = w / (ms_step * 1e-3)
     out = {
-        "metric": ("shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)"
-                   if args.config != "c3" else
-                   "shell pair-tests/s (10M halos x 3 x 100M particles x 40 shells, 1 GPU)"),
+        "metric": METRICS[args.config],
         "value": value,
         "unit": "pair-tests/s",
         "n_gpus": 1,
@@ -287,6 +305,7 @@ def main():
                          "valu_tflops_8_per_test": round(count_tflops, 3),
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
                          "hbm_gbs_algorithmic": round(count_gbs, 1)},
+        "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -128,6 +128,18 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  * Returns SJCS_ERR_INVALID for an unknown key or value. */
 int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
 
+/* Current value of an option above (0 = automatic where that applies). */
+int sjcs_get_option(sjcs_engine *e, const char *key, int64_t *value);
+
+/* Autotuner (no reference counterpart): times the cell sort + count of the
+ * chunk at d_xyz (device, n packed points) against the current halos and
+ * shells under candidate settings -- lanes per halo, cell edge along y/z as
+ * a fraction of r_box, x split, segment, tiles, by coordinate descent -- and
+ * keeps the fastest as the engine's options (sjcs_get_option reads them).
+ * The counts are not touched; the last chunk's index is discarded.
+ * *best_ms (may be NULL) receives the winner's sort + count time. */
+int sjcs_autotune(sjcs_engine *e, const float *d_xyz, uint64_t n, float *best_ms);
+
 /* Milliseconds the device spent on the last chunk, by phase (HIP events):
  * [0] upload, [1] index build (key + sort + cell table), [2] count. */
 int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);

@@ -41,7 +41,7 @@ EXPORTED = (
     "sjcs_set_shells", "sjcs_set_halos", "sjcs_set_halos_device",
     "sjcs_reset_counts", "sjcs_add_particles", "sjcs_add_particles_device",
     "sjcs_sync", "sjcs_get_counts", "sjcs_last_chunk_pair_tests",
-    "sjcs_set_option", "sjcs_last_chunk_timings",
+    "sjcs_set_option", "sjcs_get_option", "sjcs_autotune", "sjcs_last_chunk_timings",
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device",
@@ -85,6 +85,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_last_chunk_pair_tests": (i32, [vp, ctypes.POINTER(u64)]),
         "sjcs_set_option": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
         "sjcs_last_chunk_timings": (i32, [vp, ctypes.POINTER(ctypes.c_float)]),
+        "sjcs_get_option": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+        "sjcs_autotune": (i32, [vp, vp, u64, ctypes.POINTER(ctypes.c_float)]),
         "sjcs_index_create": (i32, [vp, u64, pp]),
         "sjcs_index_insert": (i32, [vp, vp, u64]),
         "sjcs_index_build": (i32, [vp]),
@@ -209,6 +211,18 @@ class Engine:
 
     def set_option(self, key: str, value: int):
         self._check(self._lib.sjcs_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key: str) -> int:
+        v = ctypes.c_int64()
+        self._check(self._lib.sjcs_get_option(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, d_xyz: int, n: int) -> float:
+        """Times candidate settings on a device-resident chunk and keeps the
+        fastest (see sjcs_autotune); returns its sort + count milliseconds."""
+        ms = ctypes.c_float()
+        self._check(self._lib.sjcs_autotune(self._h, ctypes.c_void_p(d_xyz), n, ctypes.byref(ms)))
+        return ms.value
 
 
 class ArraySTR:

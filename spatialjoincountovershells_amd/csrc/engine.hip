@@ -601,7 +601,15 @@ int prepare_counts(sjcs_engine* e) {
 
 // Lanes per halo of the count kernel: "kernel" 1 forces the plain one
 // halo per wavefront kernel, else "lanes_per_halo" (0 = automatic).
-int lanes(const sjcs_engine* e) { return e->kernel == 1 ? 64 : (int)e->lanes_per_halo; }
+// Automatic: 64 lanes when a box holds thousands of particles (the row walk
+// then dominates), else 16.
+int lanes(const sjcs_engine* e, uint64_t n) {
+  if (e->kernel == 1) return 64;
+  if (e->lanes_per_halo) return (int)e->lanes_per_halo;
+  const double side = 2.0 * (double)e->shells.view.rbox / 1000.0;
+  const double per_box = (double)n * side * side * side;
+  return !(per_box <= 4096.0) ? 64 : 16;
+}
 int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
 
 // Most images of one halo a particle can lie in: 1 unless the query box is
@@ -669,7 +677,7 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint
   HaloView hv = halo_view(e->halos);
   const Shells& sh = e->shells.view;
   if (!use_tiles)
-    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e), seg_len(e), st);
+    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e, ix.n), seg_len(e), st);
   TileView tv;
   tv.start = e->halos.tstart.as<uint32_t>();
   tv.len = e->halos.tlen.as<uint32_t>();
@@ -684,7 +692,7 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint
   hv.orig += nt;
   hv.n -= nt;
   return count_halos(ix, hv, sh, counts ? counts + nt * (uint64_t)sh.nrads : nullptr, accumulate,
-                     inbox, lanes(e), seg_len(e), st);
+                     inbox, lanes(e, ix.n), seg_len(e), st);
 }
 
 int piece_done(sjcs_engine* e, uint64_t n);
@@ -1020,6 +1028,108 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
   return SJCS_OK;
 }
 
+int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
+  if (!e || !key || !value) return SJCS_ERR_INVALID;
+  const struct {
+    const char* k;
+    int64_t v;
+  } opts[] = {{"cell_size_milli", e->cell_size_milli}, {"x_split", e->x_split},
+              {"kernel", e->kernel},                   {"lanes_per_halo", e->lanes_per_halo},
+              {"segment", e->segment},                 {"tiles", e->tiles},
+              {"graph", e->graph},                     {"fold_limit", e->fold_limit},
+              {"overlap", e->overlap},                 {"index_sort", e->index_sort}};
+  for (const auto& o : opts)
+    if (strcmp(key, o.k) == 0) {
+      *value = o.v;
+      return SJCS_OK;
+    }
+  return fail(e, SJCS_ERR_INVALID, std::string("unknown option ") + key);
+}
+
+int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if (!d_xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
+  if (n > 0x3fffffffull) return fail(e, SJCS_ERR_INVALID, "autotune: at most 2^30-1 particles");
+  rc = quiesce(e);
+  if (rc != SJCS_OK) return rc;
+  const size_t nc = (size_t)e->halos.n * (size_t)e->shells.nrads;
+  DevBuf scratch;
+  HIPCALL(e, ensure(scratch, 4 * (nc ? nc : 1)), "hipMalloc(autotune)");
+  struct Setting {
+    int64_t cell, xs, lanes, seg, tiles;
+  };
+  const float rbox = e->shells.view.rbox;
+  Setting cur{e->cell_size_milli, e->x_split, e->lanes_per_halo, e->segment, e->tiles};
+  if (e->kernel == 1) cur.lanes = 64;
+  // one timed sort + count (second of two runs) of this chunk with setting st
+  auto run = [&](const Setting& st, float* ms) -> int {
+    e->cell_size_milli = st.cell;
+    e->x_split = st.xs;
+    e->lanes_per_halo = st.lanes;
+    e->segment = st.seg;
+    e->tiles = st.tiles;
+    e->kernel = 0;
+    ++e->epoch;
+    const Grid g = choose_grid(n, rbox, e->cell_size_milli, e->x_split);
+    bool use_tiles = false;
+    int r = ensure_tiles(e, &use_tiles);
+    if (r != SJCS_OK) return r;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
+      r = build_index(e, e->ibs[0], d_xyz, n, g, e->stream);
+      if (r != SJCS_OK) return r;
+      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, scratch.as<uint32_t>(), false, nullptr,
+                             e->stream),
+              "autotune count");
+      HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
+      HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    }
+    HIPCALL(e, hipEventElapsedTime(ms, e->ev[2], e->ev[3]), "hipEventElapsedTime");
+    return SJCS_OK;
+  };
+  float best = 0.0f;
+  rc = run(cur, &best);
+  // coordinate descent: lanes, y/z cell edge (as a fraction of r_box), x
+  // split, segment, tiles
+  auto try_set = [&](Setting st) -> int {
+    float ms = 0.0f;
+    int r = run(st, &ms);
+    if (r == SJCS_OK && ms < best) {
+      best = ms;
+      cur = st;
+    }
+    return r;
+  };
+  for (int64_t l : {16, 64})
+    if (rc == SJCS_OK && l != cur.lanes) rc = try_set({cur.cell, cur.xs, l, cur.seg, cur.tiles});
+  if (rc == SJCS_OK && rbox > 0.0f && isfinite(rbox)) {
+    for (double f : {0.3, 0.45, 0.6, 0.75, 0.9, 1.2}) {
+      const int64_t milli = std::max<int64_t>(977, (int64_t)(f * rbox * 1000.0));
+      if (rc == SJCS_OK && milli != cur.cell) rc = try_set({milli, cur.xs, cur.lanes, cur.seg, cur.tiles});
+    }
+  }
+  for (int64_t x : {1, 2, 4, 8})
+    if (rc == SJCS_OK && x != cur.xs) rc = try_set({cur.cell, x, cur.lanes, cur.seg, cur.tiles});
+  for (int64_t sg : {4, 8})
+    if (rc == SJCS_OK && sg != cur.seg) rc = try_set({cur.cell, cur.xs, cur.lanes, sg, cur.tiles});
+  if (rc == SJCS_OK && e->halos.n_tiled)
+    rc = try_set({cur.cell, cur.xs, cur.lanes, cur.seg, cur.tiles ? 0 : 1});
+  // keep the winner; the index buffers of slot 0 no longer hold the last chunk
+  e->cell_size_milli = cur.cell;
+  e->x_split = cur.xs;
+  e->lanes_per_halo = cur.lanes;
+  e->segment = cur.seg;
+  e->tiles = cur.tiles;
+  e->kernel = 0;
+  ++e->epoch;
+  e->last = -1;
+  e->timed = false;
+  scratch.release();
+  if (rc == SJCS_OK && best_ms) *best_ms = best;
+  return rc;
+}
+
 int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
   if (!e || !key) return SJCS_ERR_INVALID;
   // options change which stream sorts run on and what graphs hold: let the
@@ -1205,7 +1315,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
     if (h == hipSuccess) h = ensure(ucounts, nc * sizeof(uint64_t));
     if (h == hipSuccess)
       h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint32_t>(), false,
-                      nullptr, lanes(e), seg_len(e), e->stream);
+                      nullptr, lanes(e, ix->n), seg_len(e), e->stream);
     if (h == hipSuccess)
       h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.row.as<uint32_t>(),
                          ucounts.as<uint64_t>(), n, nrads, e->stream);

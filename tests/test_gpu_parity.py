@@ -415,6 +415,38 @@ def test_halo_tiles(eng, spec):
         eng.set_option("tiles", 1)
 
 
+def test_autotune_keeps_counts(built):
+    """The autotuner picks settings from timed runs; counts are unchanged by
+    it and equal the oracle afterwards, on clustered data with tiles."""
+    rng = np.random.default_rng(27)
+    p = rng.random((80_000, 3), dtype=np.float32) * np.float32(1000.0)
+    clump = (np.float32(300.0) + rng.normal(0, 1.0, (30_000, 3))).astype(np.float32)
+    p = np.concatenate([p, clump]).astype(np.float32)
+    h = np.concatenate([np.repeat(np.array([[300.0, 300.0, 300.0]], np.float32), 12, axis=0),
+                        rng.random((1_500, 3), dtype=np.float32) * np.float32(1000.0)])
+    _, r2 = t.oracle_thresholds(0.01, 20.0, 16)
+    e = built.Engine(0)
+    try:
+        e.set_shells(r2)
+        e.set_halos(h)
+        d = e.device_array(p)
+        try:
+            e.add_particles_device(d, p.shape[0])
+            before = e.counts()
+            ms = e.autotune(d, p.shape[0])
+            assert ms > 0
+            assert np.array_equal(e.counts(), before)  # counts untouched
+            for key in ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles"):
+                e.get_option(key)
+            e.reset_counts()
+            e.add_particles_device(d, p.shape[0])
+            assert_counts_equal(e.counts(), oracle_chunks([p], h, r2)[0], h)
+        finally:
+            e.device_free(d)
+    finally:
+        e.close()
+
+
 def test_degenerate_sizes(eng):
     _, r2 = t.oracle_thresholds(0.1, 5.0, 4)
     p, h = gen_uniform(17, 5_000, 50)
