@@ -144,6 +144,45 @@ int sjcs_autotune(sjcs_engine *e, const float *d_xyz, uint64_t n, float *best_ms
  * [0] upload, [1] index build (key + sort + cell table), [2] count. */
 int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);
 
+/* ---- Serving loop (no reference counterpart) ----
+ * Requests -- a batch of halo centres with its own squared shell thresholds
+ * -- counted against one built index.  A worker thread admits queued
+ * requests in arrival order while one of `slots` streams is free and the
+ * requests in flight fit `mem_budget` device bytes; admitted requests run
+ * concurrently, and each result is independent of what else is in flight.
+ * The index must outlive the server.  All calls are thread-safe. */
+typedef struct sjcs_server sjcs_server;
+
+/* slots: concurrent requests (0 = 8); mem_budget: bytes (0 = half the free
+ * device memory at creation). */
+int sjcs_server_create(sjcs_index *ix, int slots, uint64_t mem_budget, sjcs_server **out);
+
+/* Queues a request (centres and thresholds are copied); its counts
+ * [n][nrads] (u64, caller-owned, not read) are written when it completes.
+ * *ticket names it for the calls below. */
+int sjcs_server_submit(sjcs_server *sv, const float *centrals, size_t stride_bytes, uint64_t n,
+                       const float *radiuses_sq, int nrads, uint64_t *counts, uint64_t *ticket);
+
+/* 1 = done, 0 = queued or running, < 0 = the request failed. */
+int sjcs_server_test(sjcs_server *sv, uint64_t ticket);
+
+/* Waits until the request is done; SJCS_OK or its error. */
+int sjcs_server_wait(sjcs_server *sv, uint64_t ticket);
+
+/* Seconds since the server started at which the request was submitted,
+ * admitted and done. */
+int sjcs_server_times(sjcs_server *sv, uint64_t ticket, double *submitted, double *admitted,
+                      double *done);
+
+/* Forgets a finished request. */
+int sjcs_server_release(sjcs_server *sv, uint64_t ticket);
+
+/* Message of the server's last error. */
+const char *sjcs_server_error(const sjcs_server *sv);
+
+/* Finishes every queued and running request, then frees the server. */
+void sjcs_server_destroy(sjcs_server *sv);
+
 /* Device memory helpers for callers without a HIP runtime of their own
  * (benchmarks, bindings).  No reference counterpart. */
 int sjcs_device_alloc(sjcs_engine *e, size_t bytes, void **d_ptr);

@@ -45,6 +45,8 @@ EXPORTED = (
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device",
+    "sjcs_server_create", "sjcs_server_submit", "sjcs_server_test", "sjcs_server_wait",
+    "sjcs_server_times", "sjcs_server_release", "sjcs_server_error", "sjcs_server_destroy",
 )
 
 
@@ -86,6 +88,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_set_option": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
         "sjcs_last_chunk_timings": (i32, [vp, ctypes.POINTER(ctypes.c_float)]),
         "sjcs_get_option": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+        "sjcs_server_create": (i32, [vp, i32, u64, pp]),
+        "sjcs_server_submit": (i32, [vp, vp, sz, u64, vp, i32, vp, ctypes.POINTER(u64)]),
+        "sjcs_server_test": (i32, [vp, u64]),
+        "sjcs_server_wait": (i32, [vp, u64]),
+        "sjcs_server_times": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
+        "sjcs_server_release": (i32, [vp, u64]),
+        "sjcs_server_error": (ctypes.c_char_p, [vp]),
+        "sjcs_server_destroy": (None, [vp]),
         "sjcs_autotune": (i32, [vp, vp, u64, ctypes.POINTER(ctypes.c_float)]),
         "sjcs_index_create": (i32, [vp, u64, pp]),
         "sjcs_index_insert": (i32, [vp, vp, u64]),
@@ -260,6 +272,60 @@ class ArraySTR:
         if getattr(self, "_h", None):
             self._lib.sjcs_index_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Serving loop over a built index (sjcs_server_*): submit halo batches
+    with their own shells, wait for their counts."""
+
+    def __init__(self, index: ArraySTR, slots: int = 0, mem_budget: int = 0):
+        self._ix = index
+        self._lib = index._lib
+        h = ctypes.c_void_p()
+        index._e._check(self._lib.sjcs_server_create(index._h, slots, mem_budget, ctypes.byref(h)))
+        self._h = h
+        self._live = {}  # ticket -> (centres, thresholds, counts) kept alive
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SJCSError(rc, self._lib.sjcs_server_error(self._h).decode())
+        return rc
+
+    def submit(self, centrals, radiuses_sq) -> int:
+        c = _points(centrals, "centrals")
+        r2 = _shells(radiuses_sq)
+        counts = np.zeros((c.shape[0], r2.size), dtype=np.uint64)
+        t = ctypes.c_uint64()
+        self._check(self._lib.sjcs_server_submit(self._h, _ptr(c), 12, c.shape[0], _ptr(r2),
+                                                 r2.size, _ptr(counts), ctypes.byref(t)))
+        self._live[t.value] = (c, r2, counts)
+        return t.value
+
+    def done(self, ticket: int) -> bool:
+        return self._check(self._lib.sjcs_server_test(self._h, ticket)) == 1
+
+    def wait(self, ticket: int) -> np.ndarray:
+        self._check(self._lib.sjcs_server_wait(self._h, ticket))
+        return self._live[ticket][2]
+
+    def times(self, ticket: int):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.sjcs_server_times(self._h, ticket, ctypes.byref(a), ctypes.byref(b),
+                                                ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def release(self, ticket: int):
+        self._check(self._lib.sjcs_server_release(self._h, ticket))
+        self._live.pop(ticket, None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sjcs_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
 
     def __del__(self):
         self.close()

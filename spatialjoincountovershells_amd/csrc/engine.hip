@@ -10,7 +10,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sjcs.h"
@@ -126,7 +133,8 @@ int key_bits_for(uint32_t ncells) {
 // Cell edge along y and z: 0.9 of the query half-width by default (a query
 // box then spans about 3 rows per axis, and its sphere cuts most of each
 // row), coarsened until there are at most ~2 cells per particle; about 8
-// particles per cell when the radius is not known.  At most 1024 cells per
+// particles per cell (at most 256 cells per axis) when the radius is not
+// known.  At most 1024 cells per
 // axis.  Along x the cells are `x_split` times finer (4 by default), within
 // 1024 cells and ~4 cells per particle.
 Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli, int64_t x_split) {
@@ -136,7 +144,7 @@ Grid choose_grid(uint64_t n, float rbox, int64_t cell_milli, int64_t x_split) {
   else if (rbox > 0.0f && isfinite(rbox))
     s = (double)rbox * 0.9;
   else
-    s = 1000.0 / std::max(1.0, cbrt((double)n / 8.0));
+    s = 1000.0 / std::min(256.0, std::max(1.0, cbrt((double)n / 8.0)));
   double gd = ceil(1000.0 / s);
   if (!(gd >= 1.0)) gd = 1.0;
   if (gd > 1024.0) gd = 1024.0;
@@ -300,11 +308,9 @@ int ensure_ws(sjcs_engine* e, uint64_t words) {
   return SJCS_OK;
 }
 
-// Validates and uploads shell thresholds; builds the first-match table.
-int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
-  if (!r2 || nrads <= 0) return fail(e, SJCS_ERR_INVALID, "nrads must be >= 1");
-  st.nrads = nrads;
-  st.r2.assign(r2, r2 + nrads);
+// Shell view (device pointers left null) and first-match table of nrads
+// squared thresholds; host only.
+void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry>* out_lut) {
   bool sorted = true;
   for (int k = 0; k < nrads && sorted; ++k) {
     uint32_t b = float_bits(r2[k]);
@@ -355,6 +361,18 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
       break;
     }
   }
+  *out = v;
+  out_lut->swap(lut);
+}
+
+// Validates and uploads shell thresholds; builds the first-match table.
+int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
+  if (!r2 || nrads <= 0) return fail(e, SJCS_ERR_INVALID, "nrads must be >= 1");
+  st.nrads = nrads;
+  st.r2.assign(r2, r2 + nrads);
+  Shells v{};
+  std::vector<ShellEntry> lut;
+  shell_table(r2, nrads, &v, &lut);
   HIPCALL(e, ensure(st.d_r2, sizeof(float) * nrads), "hipMalloc(shells)");
   HIPCALL(e, ensure(st.d_lut, sizeof(ShellEntry) * lut.size()), "hipMalloc(shell table)");
   HIPCALL(e, hipMemcpyAsync(st.d_r2.p, r2, sizeof(float) * nrads, hipMemcpyHostToDevice, e->stream),
@@ -1345,6 +1363,382 @@ void sjcs_index_destroy(sjcs_index* ix) {
   ix->aos.release();
   ix->ib.release();
   delete ix;
+}
+
+}  // extern "C"
+
+// ---- serving loop ----
+//
+// Requests (a halo batch with its own shells) against one built index.  A
+// worker thread admits queued requests in arrival order while a slot (own
+// stream and buffers) is free and the requests in flight fit the device
+// memory budget; admitted requests run concurrently on their streams.  The
+// worker retires finished requests (event query), copies their counts to the
+// caller's buffer and admits the next ones, so requests that arrive while
+// others run start as soon as room frees up.  Each request is computed on its
+// own, so its counts do not depend on what else is in flight.
+
+namespace {
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct ServerRequest {
+  uint64_t ticket = 0;
+  uint64_t n = 0;
+  int nrads = 0;
+  std::vector<float> xyz;  // packed halo centres
+  std::vector<float> r2;
+  Shells shells{};         // device pointers filled at launch
+  std::vector<ShellEntry> lut;
+  uint64_t* out = nullptr;
+  size_t need = 0;         // device bytes while in flight
+  int status = 1;          // 1 queued or running, 0 done, < 0 failed
+  std::string err;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+// Device bytes a request holds while in flight.
+size_t request_bytes(uint64_t n, int nrads) {
+  return 12 * (size_t)n + 2 * 20 * (size_t)n + 4 * radix_workspace_words(n) +
+         12 * (size_t)n * (size_t)nrads + 8 * (size_t)kLutSize + 4 * (size_t)nrads + 4096;
+}
+
+// Below this many halos a request is counted in input order (a sort costs
+// more launches than it saves).
+constexpr uint64_t kServeSortMin = 4096;
+
+struct ServerSlot {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  DevBuf xyz, key[2], x[2], y[2], z[2], row[2], ws, r2, lut, c32, c64;
+  void* pin = nullptr;
+  size_t pin_bytes = 0;
+  std::shared_ptr<ServerRequest> req;
+  void release() {
+    xyz.release();
+    for (int i = 0; i < 2; ++i) {
+      key[i].release();
+      x[i].release();
+      y[i].release();
+      z[i].release();
+      row[i].release();
+    }
+    ws.release();
+    r2.release();
+    lut.release();
+    c32.release();
+    c64.release();
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
+    pin_bytes = 0;
+    if (done) (void)hipEventDestroy(done);
+    if (st) (void)hipStreamDestroy(st);
+    done = nullptr;
+    st = nullptr;
+  }
+};
+
+}  // namespace
+
+struct sjcs_server {
+  sjcs_index* ix = nullptr;
+  int device = 0;
+  std::vector<ServerSlot> slots;
+  uint64_t budget = 0, used = 0;
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv;  // a request finished
+  std::deque<std::shared_ptr<ServerRequest>> queue;
+  std::unordered_map<uint64_t, std::shared_ptr<ServerRequest>> reqs;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+  std::string err;
+};
+
+namespace {
+
+#define SRVCALL(expr, what)                                          \
+  do {                                                               \
+    hipError_t h_ = (expr);                                          \
+    if (h_ != hipSuccess) {                                          \
+      r.err = std::string(what) + ": " + hipGetErrorString(h_);      \
+      return h_ == hipErrorOutOfMemory ? SJCS_ERR_NOMEM : SJCS_ERR_DEVICE; \
+    }                                                                \
+  } while (0)
+
+// Enqueues one request on its slot: upload, halo order, count, read-out in
+// input order, download into the slot's pinned buffer, event.
+int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
+  const uint64_t n = r.n;
+  const int K = r.nrads;
+  const size_t b = 4 * (size_t)(n ? n : 1);
+  const size_t nc = (size_t)n * (size_t)K;
+  SRVCALL(ensure(sl.xyz, 3 * b), "hipMalloc");
+  for (int i = 0; i < 2; ++i) {
+    SRVCALL(ensure(sl.key[i], b), "hipMalloc");
+    SRVCALL(ensure(sl.x[i], b), "hipMalloc");
+    SRVCALL(ensure(sl.y[i], b), "hipMalloc");
+    SRVCALL(ensure(sl.z[i], b), "hipMalloc");
+    SRVCALL(ensure(sl.row[i], b), "hipMalloc");
+  }
+  SRVCALL(ensure(sl.ws, 4 * radix_workspace_words(n)), "hipMalloc");
+  SRVCALL(ensure(sl.r2, 4 * (size_t)K), "hipMalloc");
+  SRVCALL(ensure(sl.lut, sizeof(ShellEntry) * r.lut.size()), "hipMalloc");
+  SRVCALL(ensure(sl.c32, 4 * (nc ? nc : 1)), "hipMalloc");
+  SRVCALL(ensure(sl.c64, 8 * (nc ? nc : 1)), "hipMalloc");
+  const size_t pin = std::max(12 * (size_t)n, 8 * nc) + 64;
+  if (sl.pin_bytes < pin) {
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl.pin = nullptr;
+    sl.pin_bytes = 0;
+    SRVCALL(hipHostMalloc(&sl.pin, pin, hipHostMallocDefault), "hipHostMalloc");
+    sl.pin_bytes = pin;
+  }
+  if (n) memcpy(sl.pin, r.xyz.data(), 12 * (size_t)n);
+  if (n)
+    SRVCALL(hipMemcpyAsync(sl.xyz.p, sl.pin, 12 * (size_t)n, hipMemcpyHostToDevice, sl.st),
+            "hipMemcpy");
+  SRVCALL(hipMemcpyAsync(sl.r2.p, r.r2.data(), 4 * (size_t)K, hipMemcpyHostToDevice, sl.st),
+          "hipMemcpy");
+  SRVCALL(hipMemcpyAsync(sl.lut.p, r.lut.data(), sizeof(ShellEntry) * r.lut.size(),
+                         hipMemcpyHostToDevice, sl.st),
+          "hipMemcpy");
+  SRVCALL(halo_keys(sl.xyz.as<float>(), n, sl.key[0].as<uint32_t>(), sl.x[0].as<float>(),
+                    sl.y[0].as<float>(), sl.z[0].as<float>(), sl.row[0].as<uint32_t>(), sl.st),
+          "halo keys");
+  int sel = 0;
+  if (n >= kServeSortMin) {
+    uint32_t* keys[2] = {sl.key[0].as<uint32_t>(), sl.key[1].as<uint32_t>()};
+    uint32_t* pay[2][4] = {{sl.x[0].as<uint32_t>(), sl.y[0].as<uint32_t>(), sl.z[0].as<uint32_t>(),
+                            sl.row[0].as<uint32_t>()},
+                           {sl.x[1].as<uint32_t>(), sl.y[1].as<uint32_t>(), sl.z[1].as<uint32_t>(),
+                            sl.row[1].as<uint32_t>()}};
+    SRVCALL(radix_sort_pairs(keys, pay, 4, n, 30, sl.ws.as<uint32_t>(), sl.st, &sel), "halo sort");
+  }
+  HaloView hv;
+  hv.x = sl.x[sel].as<float>();
+  hv.y = sl.y[sel].as<float>();
+  hv.z = sl.z[sel].as<float>();
+  hv.orig = sl.row[sel].as<uint32_t>();
+  hv.n = n;
+  r.shells.r2 = sl.r2.as<float>();
+  r.shells.lut = sl.lut.as<ShellEntry>();
+  const IndexView ix = sv->ix->ib.view();
+  const double side = 2.0 * (double)r.shells.rbox / 1000.0;
+  const int lanes_auto = !((double)ix.n * side * side * side <= 4096.0) ? 64 : 16;
+  SRVCALL(count_halos(ix, hv, r.shells, sl.c32.as<uint32_t>(), false, nullptr, lanes_auto, 4, sl.st),
+          "count");
+  SRVCALL(readout_counts(sl.c32.as<uint32_t>(), nullptr, hv.orig, sl.c64.as<uint64_t>(), n, K, sl.st),
+          "read out");
+  if (nc)
+    SRVCALL(hipMemcpyAsync(sl.pin, sl.c64.p, 8 * nc, hipMemcpyDeviceToHost, sl.st), "hipMemcpy");
+  SRVCALL(hipEventRecord(sl.done, sl.st), "hipEventRecord");
+  return SJCS_OK;
+}
+#undef SRVCALL
+
+void serve_loop(sjcs_server* sv) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    bool busy = false;
+    // retire finished requests
+    for (ServerSlot& sl : sv->slots) {
+      if (!sl.req) continue;
+      busy = true;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      ServerRequest& r = *sl.req;
+      if (q == hipSuccess) memcpy(r.out, sl.pin, 8 * (size_t)r.n * (size_t)r.nrads);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (q == hipSuccess) {
+        r.status = 0;
+      } else {
+        r.err = std::string("request: ") + hipGetErrorString(q);
+        r.status = SJCS_ERR_DEVICE;
+      }
+      r.t_done = now_s(sv->t0);
+      r.xyz.clear();
+      r.xyz.shrink_to_fit();
+      sv->used -= r.need;
+      sl.req.reset();
+      sv->cv.notify_all();
+    }
+    // admit queued requests in order while a slot and the budget allow
+    for (;;) {
+      std::shared_ptr<ServerRequest> next;
+      ServerSlot* free_slot = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->queue.empty()) break;
+        for (ServerSlot& sl : sv->slots)
+          if (!sl.req) {
+            free_slot = &sl;
+            break;
+          }
+        if (!free_slot) break;
+        const auto& f = sv->queue.front();
+        // a request larger than the whole budget still runs, alone
+        if (sv->used && sv->used + f->need > sv->budget) break;
+        next = f;
+        sv->queue.pop_front();
+        sv->used += next->need;
+        next->t_admit = now_s(sv->t0);
+      }
+      busy = true;
+      const int rc = serve_launch(sv, *free_slot, *next);
+      if (rc != SJCS_OK) {
+        (void)hipStreamSynchronize(free_slot->st);
+        std::lock_guard<std::mutex> lk(sv->mu);
+        next->status = rc;
+        next->t_done = now_s(sv->t0);
+        sv->used -= next->need;
+        sv->cv.notify_all();
+      } else {
+        free_slot->req = next;
+      }
+    }
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      bool active = false;
+      for (const ServerSlot& sl : sv->slots) active = active || (bool)sl.req;
+      if (sv->stop && sv->queue.empty() && !active) return;
+      if (!busy && sv->queue.empty())
+        sv->cv.wait_for(lk, std::chrono::milliseconds(1));  // idle: a submit wakes us
+    }
+    if (busy) std::this_thread::yield();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sjcs_server_create(sjcs_index* ix, int slots, uint64_t mem_budget, sjcs_server** out) {
+  if (!ix || !out) return SJCS_ERR_INVALID;
+  *out = nullptr;
+  sjcs_engine* e = ix->e;
+  if (!ix->built) return fail(e, SJCS_ERR_STATE, "index not built (sjcs_index_build)");
+  if (slots < 0 || slots > 64) return fail(e, SJCS_ERR_INVALID, "slots: 0 (automatic) to 64");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  size_t free_b = 0, total_b = 0;
+  HIPCALL(e, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  sjcs_server* sv = new sjcs_server();
+  sv->ix = ix;
+  sv->device = e->device;
+  sv->budget = mem_budget ? mem_budget : free_b / 2;
+  sv->t0 = std::chrono::steady_clock::now();
+  sv->slots.resize(slots ? slots : 8);
+  hipError_t h = hipSuccess;
+  for (ServerSlot& sl : sv->slots) {
+    if (h == hipSuccess) h = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+  }
+  if (h != hipSuccess) {
+    for (ServerSlot& sl : sv->slots) sl.release();
+    delete sv;
+    return hip_fail(e, h, "server streams");
+  }
+  sv->worker = std::thread(serve_loop, sv);
+  *out = sv;
+  return SJCS_OK;
+}
+
+int sjcs_server_submit(sjcs_server* sv, const float* centrals, size_t stride_bytes, uint64_t n,
+                       const float* radiuses_sq, int nrads, uint64_t* counts, uint64_t* ticket) {
+  if (!sv || !ticket) return SJCS_ERR_INVALID;
+  *ticket = 0;
+  if ((!centrals && n) || stride_bytes < 12 || (!counts && n) || !radiuses_sq || nrads <= 0) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->err = "submit: null pointer, stride < 12 or nrads < 1";
+    return SJCS_ERR_INVALID;
+  }
+  if (n > 0x3fffffffull) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->err = "submit: at most 2^30-1 halos per request";
+    return SJCS_ERR_INVALID;
+  }
+  auto r = std::make_shared<ServerRequest>();
+  r->n = n;
+  r->nrads = nrads;
+  r->xyz.resize(3 * (size_t)n);
+  const char* src = reinterpret_cast<const char*>(centrals);
+  for (uint64_t i = 0; i < n; ++i) memcpy(&r->xyz[3 * i], src + i * stride_bytes, 12);
+  r->r2.assign(radiuses_sq, radiuses_sq + nrads);
+  shell_table(radiuses_sq, nrads, &r->shells, &r->lut);
+  r->out = counts;
+  r->need = request_bytes(n, nrads);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  r->ticket = sv->next_ticket++;
+  r->t_submit = now_s(sv->t0);
+  sv->reqs[r->ticket] = r;
+  sv->queue.push_back(r);
+  *ticket = r->ticket;
+  sv->cv.notify_all();
+  return SJCS_OK;
+}
+
+int sjcs_server_test(sjcs_server* sv, uint64_t ticket) {
+  if (!sv) return SJCS_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return SJCS_ERR_INVALID;
+  const int st = it->second->status;
+  return st == 0 ? 1 : st > 0 ? 0 : st;
+}
+
+int sjcs_server_wait(sjcs_server* sv, uint64_t ticket) {
+  if (!sv) return SJCS_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return SJCS_ERR_INVALID;
+  std::shared_ptr<ServerRequest> r = it->second;
+  sv->cv.wait(lk, [&] { return r->status <= 0; });
+  if (r->status < 0) sv->err = r->err;
+  return r->status;
+}
+
+int sjcs_server_times(sjcs_server* sv, uint64_t ticket, double* submitted, double* admitted,
+                      double* done) {
+  if (!sv) return SJCS_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return SJCS_ERR_INVALID;
+  if (submitted) *submitted = it->second->t_submit;
+  if (admitted) *admitted = it->second->t_admit;
+  if (done) *done = it->second->t_done;
+  return SJCS_OK;
+}
+
+int sjcs_server_release(sjcs_server* sv, uint64_t ticket) {
+  if (!sv) return SJCS_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return SJCS_ERR_INVALID;
+  if (it->second->status > 0) return SJCS_ERR_STATE;  // still queued or running
+  sv->reqs.erase(it);
+  return SJCS_OK;
+}
+
+const char* sjcs_server_error(const sjcs_server* sv) { return sv ? sv->err.c_str() : ""; }
+
+void sjcs_server_destroy(sjcs_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+    sv->cv.notify_all();
+  }
+  if (sv->worker.joinable()) sv->worker.join();
+  (void)hipSetDevice(sv->device);
+  for (ServerSlot& sl : sv->slots) {
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    sl.release();
+  }
+  delete sv;
 }
 
 }  // extern "C"

@@ -447,6 +447,44 @@ def test_autotune_keeps_counts(built):
         e.close()
 
 
+def test_server_alone_equals_mix(built, eng):
+    """Serving loop: requests of 1 to 30000 halos, each with its own shells
+    (K 8-64, r_max 1-10, r_min = r_max/5000), give the same counts alone
+    (submitted and awaited one by one) and in the mix (all submitted at once,
+    more than slots and budget hold), equal to the oracle on the small ones."""
+    rng = np.random.default_rng(28)
+    p = rng.random((300_000, 3), dtype=np.float32) * np.float32(1000.0)
+    ix = built.ArraySTR(eng, p.shape[0])
+    ix.insert(p)
+    ix.make_tree()
+    reqs = []
+    for n in (1, 3, 64, 500, 4095, 4096, 9000, 30_000, 2, 700, 12_000, 1):
+        rmax = float(rng.uniform(1.0, 10.0))
+        k = int(rng.integers(8, 65))
+        _, r2 = t.oracle_thresholds(rmax / 5000.0, rmax, k)
+        h = rng.random((n, 3), dtype=np.float32) * np.float32(1000.0)
+        reqs.append((h, r2))
+    sv = built.Server(ix, slots=3, mem_budget=64 << 20)
+    try:
+        alone = []
+        for h, r2 in reqs:
+            tk = sv.submit(h, r2)
+            alone.append(sv.wait(tk).copy())
+            sv.release(tk)
+        tickets = [sv.submit(h, r2) for h, r2 in reqs]
+        for (h, r2), a, tk in zip(reqs, alone, tickets):
+            assert np.array_equal(sv.wait(tk), a)
+            sub, adm, done = sv.times(tk)
+            assert sub <= adm <= done
+        for (h, r2), a in zip(reqs, alone):
+            if h.shape[0] <= 700:
+                want, _ = t.oracle_count(p, h, r2, threads=8)
+                assert_counts_equal(a, want, h)
+    finally:
+        sv.close()
+        ix.close()
+
+
 def test_degenerate_sizes(eng):
     _, r2 = t.oracle_thresholds(0.1, 5.0, 4)
     p, h = gen_uniform(17, 5_000, 50)
