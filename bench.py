@@ -55,6 +55,10 @@ CONFIGS = {
                         "--radius 0.01:20:64, one chunk",
                np=200_000_000, nh=5_000_000, radius=(0.01, 20.0, 64), chunks=1,
                clustered=True, autotune=True),
+    "c5": dict(workload="serving: 3e8 uniform particles resident; requests of floor(10^U[0,5]) "
+                        "uniform halos, K ~ U{8..64}, r_max ~ U[1,10], r_min = r_max/5000; "
+                        "Poisson arrivals",
+               np=300_000_000, nh=0, radius=(0.002, 10.0, 64), chunks=1, serving=True),
     "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
                         "u64 counts accumulated on the device",
                np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3, graph=True),
@@ -66,6 +70,7 @@ METRICS = {
     "c2": "shell pair-tests/s (10M halos x 100M particles x 40 shells, 1 GPU)",
     "c3": "shell pair-tests/s (10M halos x 3 x 100M particles x 40 shells, 1 GPU)",
     "c4": "shell pair-tests/s (5M clustered halos x 200M clustered particles x 64 shells, 1 GPU)",
+    "c5": "served requests/s (3e8-particle resident index, batches of 1-1e5 halos, own shells)",
 }
 
 
@@ -139,6 +144,94 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def serve_bench(args, cfg):
+    """Config 5: a resident index and a stream of requests through the
+    serving loop.  First the capacity (all requests submitted at once), then
+    Poisson arrivals at `load` times that rate: latency p50/p99 from the
+    server's own submit and completion times."""
+    rng = np.random.default_rng(args.seed)
+    log(f"[bench] generating {cfg['np']} particles")
+    parts = synth.uniform_points(rng, cfg["np"])
+    reqs = []
+    for _ in range(args.requests):
+        n = int(10 ** rng.uniform(0.0, 5.0))
+        k = int(rng.integers(8, 65))
+        rmax = float(rng.uniform(1.0, 10.0))
+        _, r2 = sj.shell_thresholds(rmax / 5000.0, rmax, k)
+        reqs.append((synth.uniform_points(rng, n), r2))
+    halos = sum(h.shape[0] for h, _ in reqs)
+    eng = sj.Engine(0)
+    ix = sj.ArraySTR(eng, parts.shape[0])
+    t0 = time.perf_counter()
+    ix.insert(parts)
+    ix.make_tree()
+    build_s = time.perf_counter() - t0
+    del parts
+    sv = sj.Server(ix)
+    # warm-up: the slots' buffers grow to the largest request
+    for h, r2 in sorted(reqs, key=lambda q: -q[0].shape[0])[:16]:
+        tk = sv.submit(h, r2)
+        sv.wait(tk)
+        sv.release(tk)
+    log("[bench] capacity run")
+    t0 = time.perf_counter()
+    tickets = [sv.submit(h, r2) for h, r2 in reqs]
+    for tk in tickets:
+        sv.wait(tk)
+    cap_s = time.perf_counter() - t0
+    capacity = len(reqs) / cap_s
+    for tk in tickets:
+        sv.release(tk)
+    rate = args.load * capacity
+    log(f"[bench] capacity {capacity:.1f} req/s; Poisson arrivals at {rate:.1f} req/s")
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    t0 = time.perf_counter()
+    due = 0.0
+    tickets = []
+    for (h, r2), g in zip(reqs, gaps):
+        due += g
+        wait = due - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        tickets.append(sv.submit(h, r2))
+    lat = []
+    for tk in tickets:
+        sv.wait(tk)
+        sub, adm, done = sv.times(tk)
+        lat.append(done - sub)
+    run_s = time.perf_counter() - t0
+    lat = np.array(lat) * 1e3
+    sv.close()
+    ix.close()
+    eng.close()
+    out = {
+        "metric": METRICS[args.config],
+        "value": capacity,
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 16,
+        "ms_per_step": 1e3 / capacity,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "particles": cfg["np"], "requests": len(reqs),
+                   "halos": halos, "slots": 8, "parallelism": "single GPU"},
+        "capacity": {"requests_per_s": capacity, "halos_per_s": halos / cap_s,
+                     "wall_s": round(cap_s, 3)},
+        "poisson": {"rate_per_s": rate, "load": args.load, "wall_s": round(run_s, 3),
+                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                   "p99": float(np.percentile(lat, 99)),
+                                   "max": float(lat.max())}},
+        "index_build_s": round(build_s, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +253,9 @@ def main():
     ap.add_argument("--cpu-halos", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
+    ap.add_argument("--requests", type=int, default=2000, help="serving config: requests")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="serving config: arrival rate as a fraction of the measured capacity")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -176,6 +272,8 @@ def main():
         sys.exit(2)
 
     cfg = CONFIGS[args.config]
+    if cfg.get("serving"):
+        return serve_bench(args, cfg)
     mn, mx, k = cfg["radius"]
     _, r2 = sj.shell_thresholds(mn, mx, k)
     log(f"[bench] generating {cfg['workload']}")
