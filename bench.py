@@ -42,6 +42,7 @@ import spatialjoincountovershells_amd as sj  # noqa: E402
 from spatialjoincountovershells_amd import synth  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "astr_rcount")
+CLI_BIN = os.path.join(ROOT, "spatialjoincountovershells_amd", "bin", "sjcs_rcount")
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -89,6 +90,60 @@ def make_inputs(cfg, seed):
     for c in range(1, cfg["chunks"]):
         parts.append(synth.uniform_points(np.random.default_rng(seed + 1000 * c), cfg["np"]))
     return parts, halos
+
+
+def write_halo_file(path, halos):
+    """The halo text format the reference reads (src/Gadget/BufHalo.c:76-119):
+    16 header lines, then id and 7 other columns with x, y, z in columns 9-11;
+    %.9g gives back the same FP32 values through strtof."""
+    with open(path, "w") as f:
+        for i in range(16):
+            f.write(f"# header {i}\n")
+        step = 1 << 20
+        for a in range(0, halos.shape[0], step):
+            blk = halos[a:a + step].tolist()
+            f.write("".join(f"{a + i} 0 0 0 0 0 0 0 {x:.9g} {y:.9g} {z:.9g} 0\n"
+                            for i, (x, y, z) in enumerate(blk)))
+
+
+def timer_lines(text):
+    return {m.group(1): int(m.group(2))
+            for m in re.finditer(r">>>> ([a-z ]+) time: (\d+) milli seconds", text)}
+
+
+def wall_legs(parts, halos, radius, threads, with_reference):
+    """End-to-end process wall time of the drop-in CLI (and, if asked, of the
+    reference binary built from its sources) on files written here: reading
+    the halo text and the particle files, the count, formatting the result
+    (to /dev/null: disk speed is not measured)."""
+    tmp = tempfile.mkdtemp(prefix="sjcs_wall_")
+    try:
+        pfs = []
+        for c, p in enumerate(parts):
+            pfs.append(os.path.join(tmp, f"part{c}.dat"))
+            p.tofile(pfs[-1])
+        hf = os.path.join(tmp, "halo.list")
+        write_halo_file(hf, halos)
+        mn, mx, k = radius
+        args = ["--thread", str(threads), "--next-setup", str(threads), "--particle-files", *pfs,
+                "--halo-file", hf, "--radius", f"{mn}:{mx}:{k}", "--output-file", os.devnull]
+        res = {}
+        legs = [("sjcs_rcount", CLI_BIN)]
+        if with_reference and os.path.exists(REF_BIN):
+            legs.append(("reference astr_rcount", REF_BIN))
+        for name, exe in legs:
+            extra = ["--backlog", "0"] if exe == REF_BIN else []
+            t0 = time.time()
+            out = subprocess.run([exe] + extra + args, capture_output=True, text=True, timeout=900)
+            wall = time.time() - t0
+            if out.returncode != 0:
+                log(f"{name} failed rc={out.returncode}: {out.stderr[-300:]}")
+                continue
+            res[name] = {"wall_s": round(wall, 3), "phases_ms": timer_lines(out.stdout),
+                         "threads": threads}
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
@@ -254,6 +309,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--requests", type=int, default=2000, help="serving config: requests")
+    ap.add_argument("--wall", action="store_true",
+                    help="also time the drop-in CLI end to end on files of this workload")
+    ap.add_argument("--wall-reference", action="store_true",
+                    help="with --wall: time the reference binary on the same files too")
     ap.add_argument("--load", type=float, default=0.7,
                     help="serving config: arrival rate as a fraction of the measured capacity")
     args = ap.parse_args()
@@ -377,6 +436,10 @@ def main():
     for d in d_parts:
         eng.device_free(d)
     eng.close()
+    wall = None
+    if args.wall:
+        log("[bench] end-to-end wall time of the CLI")
+        wall = wall_legs(parts, halos, cfg["radius"], args.cpu_threads, args.wall_reference)
     value = w / (ms_step * 1e-3)
     out = {
         "metric": METRICS[args.config],
@@ -404,6 +467,7 @@ def main():
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
                          "hbm_gbs_algorithmic": round(count_gbs, 1)},
         "autotune": tuned,
+        "wall": wall,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
