@@ -310,7 +310,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--requests", type=int, default=2000, help="serving config: requests")
     ap.add_argument("--wall", action="store_true",
-                    help="also time the drop-in CLI end to end on files of this workload")
+                    help="also time the drop-in CLI end to end on files of this workload "
+                         "(default for c2)")
+    ap.add_argument("--no-wall", action="store_true")
     ap.add_argument("--wall-reference", action="store_true",
                     help="with --wall: time the reference binary on the same files too")
     ap.add_argument("--load", type=float, default=0.7,
@@ -437,7 +439,7 @@ def main():
         eng.device_free(d)
     eng.close()
     wall = None
-    if args.wall:
+    if (args.wall or args.config == "c2") and not args.no_wall:
         log("[bench] end-to-end wall time of the CLI")
         wall = wall_legs(parts, halos, cfg["radius"], args.cpu_threads, args.wall_reference)
     value = w / (ms_step * 1e-3)
