@@ -89,6 +89,74 @@ int sjcs_reset_counts(sjcs_engine *e);
 int sjcs_add_particles(sjcs_engine *e, const float *xyz, uint64_t n);
 int sjcs_add_particles_device(sjcs_engine *e, const float *d_xyz, uint64_t n);
 
+/* ---- compressed (Gadget) particle files ----
+ * A compressed particle file (`--compressed`, src/Gadget/BufVector3D.c:
+ * 111-136) holds a 4-byte record marker, the 256-byte header below, 8 bytes
+ * of record markers, then npart[1] triplets of uint16 displacements.  The
+ * file's particle n has id = id_start + n; the id names a point of a regular
+ * grid (src/Gadget/GadId.c:70-86, 112-137) and the triplet displaces it
+ * (decodeCompressedParticle, BufVector3D.c:168-201):
+ *   pos[d] = cell_interval * ip[d] + disp_min + unitsep * uv[d]. */
+
+/* The file header, byte for byte (include/Gadget/GadgetHeader.h:4-27). */
+typedef struct sjcs_gadget_header {
+    int32_t npart[6];
+    double mass[6];
+    double time;
+    double redshift;
+    int32_t flag_sfr;
+    int32_t flag_feedback;
+    int32_t npartTotal[6];
+    int32_t flag_cooling;
+    int32_t num_files;
+    double BoxSize;
+    double Omega0;
+    double OmegaLambda;
+    double HubbleParam;
+    int32_t flag_stellarage;
+    int32_t flag_metals;
+    int32_t hashsize;
+    float disp_min;
+    float disp_max;
+    int32_t pos_bits;
+    int64_t id_start;
+    int64_t id_end;
+    char fill[56];
+} sjcs_gadget_header;
+
+/* What the decode needs: GadgetDecodeParams + GadId
+ * (include/Gadget/GadgetHeader.h:30-37, include/Gadget/GadId.h:5-14). */
+typedef struct sjcs_gadget_decode {
+    int64_t id_start;     /* id of the file's first particle */
+    int64_t ng1d;         /* grid points per tile side (nparticles_partile) */
+    int32_t nt1d;         /* tiles per side (astr_rcount uses 1) */
+    int32_t fused;        /* 1: cell_interval*ip + disp_min is one FMA, as gcc
+                             contracts it for an FMA-capable -march (the
+                             reference's -march=native); 0: two roundings */
+    float cell_interval;
+    float disp_min;
+    double unitsep;
+} sjcs_gadget_decode;
+
+/* initGadgetDecodeParams + initGadId (GadgetHeader.c:7-24, GadId.c:38-45,
+ * astr_rcount/main.c:694-696) for `ntile` tiles per side, fused = 1. */
+int sjcs_gadget_decode_params(const sjcs_gadget_header *hdr, int ntile,
+                              sjcs_gadget_decode *out);
+
+/* One compressed particle file's triplets (uv[3n], host) as one particle
+ * chunk: uploaded as 6 bytes per particle, decoded on the device, then
+ * indexed and counted as sjcs_add_particles does.  Replaces the compressed
+ * branch of constructIndexWithSameThreadPool (astr_rcount/main.c:739-763).
+ * SJCS_ERR_INVALID when an id of the file is out of the range the decode
+ * handles exactly (grid points beyond +-2^31, |id| >= 2^50). */
+int sjcs_add_compressed_particles(sjcs_engine *e, const uint16_t *uv, uint64_t n,
+                                  const sjcs_gadget_decode *p);
+
+/* Decodes n triplets at d_uv (device) into packed xyz at d_xyz (device);
+ * returns when done.  The decode alone, for callers that keep the points. */
+int sjcs_decode_compressed_device(sjcs_engine *e, const uint16_t *d_uv, uint64_t n,
+                                  const sjcs_gadget_decode *p, float *d_xyz);
+
 /* Waits for all queued work of this engine. */
 int sjcs_sync(sjcs_engine *e);
 
@@ -188,6 +256,7 @@ void sjcs_server_destroy(sjcs_server *sv);
 int sjcs_device_alloc(sjcs_engine *e, size_t bytes, void **d_ptr);
 int sjcs_device_free(sjcs_engine *e, void *d_ptr);
 int sjcs_copy_to_device(sjcs_engine *e, void *d_dst, const void *src, size_t bytes);
+int sjcs_copy_to_host(sjcs_engine *e, void *dst, const void *d_src, size_t bytes);
 
 /* ---- index: the ArraySTR lifecycle, one device-resident particle index ---- */
 

@@ -33,6 +33,14 @@
  *   shell        src/objects/ArraySTR/MultiRangeCount.c:107-113: first k
  *                with dist < R2[k]; nothing counted when none matches.
  *
+ *   compressed   src/Gadget/GadgetHeader.c:7-24 (decode parameters),
+ *   particles    src/Gadget/GadId.c:38-45, 70-86, 112-137 (id -> grid point,
+ *                C's truncating division), src/Gadget/BufVector3D.c:168-201
+ *                (the non-AVX branch, the one the reference's build takes:
+ *                nothing defines USE_AVX), with cell_interval*ip + disp_min
+ *                fused into fmaf() as gcc -std=gnu99 contracts it for an FMA
+ *                host, or left as two roundings (fused = 0).
+ *
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  The
  * -ffp-contract=off is what keeps the distance free of FMAs.
  */
@@ -247,4 +255,81 @@ uint64_t oracle_count(const float *particles, uint64_t np,
     free(jobs);
     free(th);
     return total;
+}
+
+/* ---- compressed (Gadget) particles ---- */
+
+/* Header fields by byte offset (include/Gadget/GadgetHeader.h:4-27: six
+ * int npart, six double mass, time, redshift, two int flags, six int
+ * npartTotal, two int, four double from BoxSize, three int, float disp_min,
+ * float disp_max, int pos_bits, long long id_start, ...). */
+#define GH_NPART_TOTAL 96
+#define GH_BOXSIZE 128
+#define GH_DISP_MIN 172
+#define GH_DISP_MAX 176
+#define GH_POS_BITS 180
+#define GH_ID_START 184
+
+void oracle_gadget_params(const unsigned char *hdr, int ntile, long long *id_start,
+                          long long *ng1d, int *nt1d, float *cell_interval,
+                          float *disp_min, double *unitsep)
+{
+    int tot[6], bits;
+    double box;
+    float dmin, dmax;
+    long long total;
+    int at1d;
+
+    memcpy(tot, hdr + GH_NPART_TOTAL, sizeof(tot));
+    memcpy(&box, hdr + GH_BOXSIZE, sizeof(box));
+    memcpy(&dmin, hdr + GH_DISP_MIN, sizeof(dmin));
+    memcpy(&dmax, hdr + GH_DISP_MAX, sizeof(dmax));
+    memcpy(&bits, hdr + GH_POS_BITS, sizeof(bits));
+    memcpy(id_start, hdr + GH_ID_START, sizeof(*id_start));
+    total = (long long)tot[1] + (long long)((unsigned long long)(long long)tot[2] << 32);
+    at1d = (int)pow((double)total + 0.5, 1 / 3.);
+    *cell_interval = (float)(box / (double)at1d);
+    *ng1d = at1d / ntile;
+    *nt1d = ntile;
+    *disp_min = dmin;
+    *unitsep = (double)(float)(dmax - dmin) / pow(2, bits);
+}
+
+/* id2posGadId: id - 1 splits into a tile number and a number within the
+ * tile; each splits into (slowest, middle, fastest) digits; axis d's grid
+ * coordinate is tile digit d * points per side + point digit d. */
+void oracle_gadget_gridpos(long long id, long long ng1d, int nt1d, int ip[3])
+{
+    const long long ng2d = (long long)(int)(ng1d * ng1d);   /* an int there */
+    const long long ng3d = ng1d * ng1d * ng1d;
+    const int nt2d = nt1d * nt1d;
+    const long long tile = (id - 1) / ng3d;
+    const long long inner = id - 1 - tile * ng3d;
+    int g[3], t[3], d;
+
+    g[0] = (int)(inner / ng2d);
+    g[1] = (int)((inner - g[0] * ng2d) / ng1d);
+    g[2] = (int)(inner - g[0] * ng2d - g[1] * ng1d);
+    t[0] = (int)(tile / nt2d);
+    t[1] = (int)((tile - t[0] * nt2d) / nt1d);
+    t[2] = (int)(tile - t[0] * nt2d - t[1] * nt1d);
+    for (d = 0; d < 3; d++)
+        ip[d] = (int)(t[d] * ng1d + g[d]);
+}
+
+void oracle_gadget_decode(const uint16_t *uv, uint64_t n, long long id_start, long long ng1d,
+                          int nt1d, float cell_interval, float disp_min, double unitsep,
+                          int fused, float *xyz)
+{
+    uint64_t i;
+    int d, ip[3];
+
+    for (i = 0; i < n; i++) {
+        oracle_gadget_gridpos((long long)((unsigned long long)id_start + i), ng1d, nt1d, ip);
+        for (d = 0; d < 3; d++) {
+            float s = fused ? fmaf(cell_interval, (float)ip[d], disp_min)
+                            : cell_interval * (float)ip[d] + disp_min;
+            xyz[3 * i + d] = (float)((double)s + unitsep * (double)uv[3 * i + d]);
+        }
+    }
 }

@@ -21,4 +21,12 @@ uint64_t oracle_count(const float *particles, uint64_t np,
                       const float *r2, int nrads, uint64_t *counts,
                       int nthreads);
 
+void oracle_gadget_params(const unsigned char *hdr, int ntile, long long *id_start,
+                          long long *ng1d, int *nt1d, float *cell_interval,
+                          float *disp_min, double *unitsep);
+void oracle_gadget_gridpos(long long id, long long ng1d, int nt1d, int ip[3]);
+void oracle_gadget_decode(const uint16_t *uv, uint64_t n, long long id_start, long long ng1d,
+                          int nt1d, float cell_interval, float disp_min, double unitsep,
+                          int fused, float *xyz);
+
 #endif

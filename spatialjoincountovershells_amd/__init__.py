@@ -44,7 +44,9 @@ EXPORTED = (
     "sjcs_set_option", "sjcs_get_option", "sjcs_autotune", "sjcs_last_chunk_timings",
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
-    "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device",
+    "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
+    "sjcs_gadget_decode_params", "sjcs_add_compressed_particles",
+    "sjcs_decode_compressed_device",
     "sjcs_server_create", "sjcs_server_submit", "sjcs_server_test", "sjcs_server_wait",
     "sjcs_server_times", "sjcs_server_release", "sjcs_server_error", "sjcs_server_destroy",
 )
@@ -56,6 +58,51 @@ class SJCSError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"sjcs error {code}: {msg}")
         self.code = code
+
+
+class GadgetDecode(ctypes.Structure):
+    """sjcs_gadget_decode: what decoding one compressed particle file needs
+    (GadgetDecodeParams + GadId, include/Gadget/GadgetHeader.h:30-37,
+    include/Gadget/GadId.h:5-14)."""
+    _fields_ = [("id_start", ctypes.c_int64), ("ng1d", ctypes.c_int64),
+                ("nt1d", ctypes.c_int32), ("fused", ctypes.c_int32),
+                ("cell_interval", ctypes.c_float), ("disp_min", ctypes.c_float),
+                ("unitsep", ctypes.c_double)]
+
+
+GADGET_HEADER_BYTES = 256
+
+
+def read_compressed(path: str):
+    """(header bytes, uint16 triplets [n][3]) of a compressed particle file:
+    4-byte marker, 256-byte header, 8 bytes of markers, npart[1] triplets
+    (src/Gadget/BufVector3D.c:111-136)."""
+    with open(path, "rb") as f:
+        f.seek(4)
+        hdr = f.read(GADGET_HEADER_BYTES)
+        if len(hdr) != GADGET_HEADER_BYTES:
+            raise ValueError(f"{path}: shorter than a Gadget header")
+        n = int(np.frombuffer(hdr, "<i4", 1, 4)[0])
+        if n < 0:
+            raise ValueError(f"{path}: npart[1] = {n}")
+        f.seek(8, os.SEEK_CUR)
+        uv = np.frombuffer(f.read(6 * n), "<u2")
+        if uv.size != 3 * n:
+            raise ValueError(f"{path}: shorter than its header says")
+    return hdr, uv.reshape(n, 3)
+
+
+def gadget_decode_params(header: bytes, ntile: int = 1) -> GadgetDecode:
+    """sjcs_gadget_decode_params of a 256-byte header (host only)."""
+    if len(header) != GADGET_HEADER_BYTES:
+        raise ValueError("a Gadget header is 256 bytes")
+    lib = load_library()
+    buf = ctypes.create_string_buffer(bytes(header), GADGET_HEADER_BYTES)
+    out = GadgetDecode()
+    rc = lib.sjcs_gadget_decode_params(buf, ntile, ctypes.byref(out))
+    if rc != SJCS_OK:
+        raise SJCSError(rc, "bad Gadget header (negative particle total)")
+    return out
 
 
 _lib = None
@@ -107,6 +154,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_device_alloc": (i32, [vp, sz, pp]),
         "sjcs_device_free": (i32, [vp, vp]),
         "sjcs_copy_to_device": (i32, [vp, vp, vp, sz]),
+        "sjcs_copy_to_host": (i32, [vp, vp, vp, sz]),
+        "sjcs_gadget_decode_params": (i32, [vp, i32, ctypes.POINTER(GadgetDecode)]),
+        "sjcs_add_compressed_particles": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode)]),
+        "sjcs_decode_compressed_device": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -190,6 +241,30 @@ class Engine:
 
     def add_particles_device(self, ptr: int, n: int):
         self._check(self._lib.sjcs_add_particles_device(self._h, ctypes.c_void_p(ptr), n))
+
+    def add_compressed_particles(self, uv, params: GadgetDecode):
+        """One compressed particle file's triplets as a chunk, decoded on
+        the device (sjcs_add_compressed_particles)."""
+        uv = np.ascontiguousarray(uv, dtype=np.uint16).reshape(-1, 3)
+        self._check(self._lib.sjcs_add_compressed_particles(self._h, _ptr(uv), uv.shape[0],
+                                                            ctypes.byref(params)))
+
+    def decode_compressed(self, uv, params: GadgetDecode) -> np.ndarray:
+        """The device decode alone: triplets in, packed xyz out."""
+        uv = np.ascontiguousarray(uv, dtype=np.uint16).reshape(-1, 3)
+        n = uv.shape[0]
+        out = np.zeros((n, 3), np.float32)
+        d_uv = self.device_array(uv if n else np.zeros(3, np.uint16))
+        d_xyz = self.device_array(out if n else np.zeros(3, np.float32))
+        try:
+            self._check(self._lib.sjcs_decode_compressed_device(
+                self._h, ctypes.c_void_p(d_uv), n, ctypes.byref(params), ctypes.c_void_p(d_xyz)))
+            self._check(self._lib.sjcs_copy_to_host(self._h, _ptr(out), ctypes.c_void_p(d_xyz),
+                                                    out.nbytes))
+        finally:
+            self.device_free(d_uv)
+            self.device_free(d_xyz)
+        return out
 
     def sync(self):
         self._check(self._lib.sjcs_sync(self._h))

@@ -23,6 +23,7 @@
 #include "../../include/sjcs.h"
 #include "common.h"
 #include "count.h"
+#include "gadget.h"
 #include "primitives.h"
 #include "tile.h"
 
@@ -208,6 +209,7 @@ struct sjcs_engine {
   // only for the count that last read the slot ("overlap" 1).
   IndexBuffers ibs[2];
   DevBuf cin[2];              // packed chunk staged on the device (host uploads)
+  DevBuf cuv[2];              // compressed triplets of a chunk, before the decode
   void* cpin[2] = {nullptr, nullptr};  // pinned host staging of a chunk
   size_t cpin_bytes[2] = {0, 0};
   int slot = 0;               // slot of the next chunk
@@ -831,6 +833,34 @@ int check_ready(sjcs_engine* e) {
   return use_device(e);
 }
 
+// Stages `bytes` of host data of the next chunk in its slot's pinned buffer
+// and queues their upload into the slot's device buffer dst on bstream, after
+// the slot's last sort (graph mode: its last graph, on the count stream) has
+// read the slot's device staging.  Records ev[0] before the upload.
+int stage_chunk(sjcs_engine* e, const void* src, size_t bytes, DevBuf& dst) {
+  const int sl = e->slot;
+  const size_t pin = bytes ? bytes : 16;
+  // the slot's pinned buffer is free once its last upload is done; its
+  // device staging once the sort that read it is done
+  HIPCALL(e, hipEventSynchronize(e->uploaded[sl]), "hipEventSynchronize");
+  if (e->cpin_bytes[sl] < pin) {
+    if (e->cpin[sl]) (void)hipHostFree(e->cpin[sl]);
+    e->cpin[sl] = nullptr;
+    e->cpin_bytes[sl] = 0;
+    HIPCALL(e, hipHostMalloc(&e->cpin[sl], pin, hipHostMallocDefault), "hipHostMalloc");
+    e->cpin_bytes[sl] = pin;
+  }
+  if (dst.bytes < bytes) HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+  HIPCALL(e, ensure(dst, bytes), "hipMalloc(points)");
+  if (bytes) memcpy(e->cpin[sl], src, bytes);
+  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
+  if (bytes)
+    HIPCALL(e, hipMemcpyAsync(dst.p, e->cpin[sl], bytes, hipMemcpyHostToDevice, e->bstream),
+            "hipMemcpy(points)");
+  return SJCS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -889,6 +919,7 @@ void sjcs_engine_destroy(sjcs_engine* e) {
       if (e->graphs[i][j].exec) (void)hipGraphExecDestroy(e->graphs[i][j].exec);
     e->ibs[i].release();
     e->cin[i].release();
+    e->cuv[i].release();
     if (e->cpin[i]) (void)hipHostFree(e->cpin[i]);
     if (e->built[i]) (void)hipEventDestroy(e->built[i]);
     if (e->counted[i]) (void)hipEventDestroy(e->counted[i]);
@@ -959,31 +990,50 @@ int sjcs_add_particles(sjcs_engine* e, const float* xyz, uint64_t n) {
   if (rc != SJCS_OK) return rc;
   if (!xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
   const int sl = e->slot;
-  const size_t bytes = 12 * (size_t)(n ? n : 1);
-  // the slot's pinned buffer is free once its last upload is done; its
-  // device staging once the sort that read it is done
-  HIPCALL(e, hipEventSynchronize(e->uploaded[sl]), "hipEventSynchronize");
-  if (e->cpin_bytes[sl] < bytes) {
-    if (e->cpin[sl]) (void)hipHostFree(e->cpin[sl]);
-    e->cpin[sl] = nullptr;
-    e->cpin_bytes[sl] = 0;
-    HIPCALL(e, hipHostMalloc(&e->cpin[sl], bytes, hipHostMallocDefault), "hipHostMalloc");
-    e->cpin_bytes[sl] = bytes;
-  }
-  if (e->cin[sl].bytes < bytes) HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
-  HIPCALL(e, ensure(e->cin[sl], bytes), "hipMalloc(points)");
-  if (n) memcpy(e->cpin[sl], xyz, 12 * (size_t)n);
-  // the staging is rewritten once the sort that read it (graph mode: the
-  // slot's last graph, on the count stream) is done
-  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
-  HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
-  if (n)
-    HIPCALL(e, hipMemcpyAsync(e->cin[sl].p, e->cpin[sl], 12 * (size_t)n, hipMemcpyHostToDevice,
-                              e->bstream),
-            "hipMemcpy(points)");
+  const size_t bytes = 12 * (size_t)n;
+  rc = stage_chunk(e, xyz, bytes, e->cin[sl]);
+  if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
   return add_chunk(e, e->cin[sl].as<float>(), n);
+}
+
+int sjcs_add_compressed_particles(sjcs_engine* e, const uint16_t* uv, uint64_t n,
+                                  const sjcs_gadget_decode* p) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if ((!uv && n) || !p) return fail(e, SJCS_ERR_INVALID, "compressed particles: null pointer");
+  GadgetDecode d;
+  std::string err;
+  if (!gadget_prepare(*p, n, &d, &err)) return fail(e, SJCS_ERR_INVALID, err);
+  const int sl = e->slot;
+  // the slot's triplets are rewritten once its last decode (bstream, before
+  // `uploaded`) has read them
+  rc = stage_chunk(e, uv, 6 * (size_t)n, e->cuv[sl]);
+  if (rc != SJCS_OK) return rc;
+  if (e->cin[sl].bytes < 12 * (size_t)n)
+    HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+  HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
+  HIPCALL(e, decode_gadget(e->cuv[sl].as<uint16_t>(), n, d, e->cin[sl].as<float>(), e->bstream),
+          "decode compressed particles");
+  HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
+  HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
+  return add_chunk(e, e->cin[sl].as<float>(), n);
+}
+
+int sjcs_decode_compressed_device(sjcs_engine* e, const uint16_t* d_uv, uint64_t n,
+                                  const sjcs_gadget_decode* p, float* d_xyz) {
+  if (!e) return SJCS_ERR_INVALID;
+  if ((!d_uv || !d_xyz) && n) return fail(e, SJCS_ERR_INVALID, "decode: null pointer");
+  if (!p) return fail(e, SJCS_ERR_INVALID, "decode: null parameters");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  GadgetDecode d;
+  std::string err;
+  if (!gadget_prepare(*p, n, &d, &err)) return fail(e, SJCS_ERR_INVALID, err);
+  HIPCALL(e, decode_gadget(d_uv, n, d, d_xyz, e->stream), "decode compressed particles");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  return SJCS_OK;
 }
 
 int sjcs_add_particles_device(sjcs_engine* e, const float* d_xyz, uint64_t n) {
@@ -1250,6 +1300,15 @@ int sjcs_copy_to_device(sjcs_engine* e, void* d_dst, const void* src, size_t byt
   if (rc != SJCS_OK) return rc;
   if (bytes == 0) return SJCS_OK;
   HIPCALL(e, hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  return SJCS_OK;
+}
+
+int sjcs_copy_to_host(sjcs_engine* e, void* dst, const void* d_src, size_t bytes) {
+  if (!e || (!dst && bytes) || (!d_src && bytes)) return SJCS_ERR_INVALID;
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (bytes == 0) return SJCS_OK;
+  HIPCALL(e, hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
   return SJCS_OK;
 }
 

@@ -11,6 +11,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include "sjcs.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -100,6 +102,17 @@ void sjcs_free_halos(sjcs_halos *h);
 int sjcs_particle_count(const char *fname, uint64_t *n);
 /* Reads the first n particles (3 x FP32 each) into dst. */
 int sjcs_read_particles(const char *fname, float *dst, uint64_t n);
+
+/* ---- compressed particles (src/Gadget/BufVector3D.c:111-136) ---- */
+
+/* Reads a compressed particle file: its header into hdr and its npart[1]
+ * uint16 triplets into *uv (grown with realloc when *cap, in particles, is
+ * smaller; the caller frees it), *n = npart[1].  Returns 0; -1 when the file
+ * cannot be opened; -2 when it is shorter than its header says (the
+ * reference then decodes whatever the buffer held) or npart[1] < 0; -3 when
+ * memory runs out. */
+int sjcs_read_compressed(const char *fname, sjcs_gadget_header *hdr, uint16_t **uv,
+                         uint64_t *cap, uint64_t *n);
 
 /* ---- result (src/astr_rcount/main.c:371-442) ---- */
 

@@ -9,7 +9,10 @@
  * from disk while the GPU works on the current one.  The per-file
  * "index construction" line reports upload + cell sort, "searching" the
  * count kernel, both from HIP events; the index buffers are reused from
- * file to file, so "index destruction" is ~0.
+ * file to file, so "index destruction" is ~0.  With --compressed, each
+ * file's uint16 triplets are uploaded as they are and decoded on the
+ * device ("particles decoding + index construction" = upload + decode +
+ * cell sort).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -37,9 +40,40 @@ static void die_engine(sjcs_engine *e, const char *what)
 
 typedef struct {
     float *xyz;
+    uint16_t *uv;             /* --compressed: the file's triplets */
+    sjcs_gadget_decode dec;   /* --compressed: their decode parameters */
     uint64_t n, cap;
     long read_ms;
 } chunk_buf;
+
+/* Reads compressed particle file fname into c (main.c:739-756). */
+static void read_compressed_chunk(const char *fname, chunk_buf *c)
+{
+    struct timeval a, b;
+    sjcs_gadget_header hdr;
+    uint64_t n = 0;
+    int rc;
+    TIMER_START(a);
+    rc = sjcs_read_compressed(fname, &hdr, &c->uv, &c->cap, &n);
+    if (rc == -1) {
+        fprintf(stderr, "ERROR: file %s not found in %s()\n", fname, "readCompressedParticles");
+        exit(EXIT_FAILURE);
+    }
+    if (rc != 0) {
+        fprintf(stderr, "ERROR: %s in %s()\n",
+                rc == -3 ? "memory shortage" : "file shorter than its header says",
+                "readCompressedParticles");
+        exit(EXIT_FAILURE);
+    }
+    /* one tile per side, as constructIndex* fixes it (main.c:744) */
+    if (sjcs_gadget_decode_params(&hdr, 1, &c->dec) != SJCS_OK) {
+        fprintf(stderr, "ERROR: bad particle totals in the header of %s\n", fname);
+        exit(EXIT_FAILURE);
+    }
+    c->n = n;
+    gettimeofday(&b, NULL);
+    c->read_ms = sjcs_elapsed_ms(&a, &b);
+}
 
 /* Reads particle file fname into c; exits like the reference would fail. */
 static void read_chunk(const char *fname, chunk_buf *c)
@@ -70,17 +104,27 @@ static void read_chunk(const char *fname, chunk_buf *c)
 }
 
 /* Waits for the chunk in flight and prints its lines. */
-static void finish_chunk(sjcs_engine *e, uint64_t n)
+static void finish_chunk(sjcs_engine *e, uint64_t n, int compressed)
 {
     float ms[3];
     struct timeval a;
     if (sjcs_last_chunk_timings(e, ms) != SJCS_OK)
         die_engine(e, "search");
-    printf(">>>> index construction time: %ld milli seconds\n", (long)(ms[0] + ms[1]));
+    printf(">>>> %s time: %ld milli seconds\n",
+           compressed ? "particles decoding + index construction" : "index construction",
+           (long)(ms[0] + ms[1]));
     printf("  # of particles = %llu\n", (unsigned long long)n);
     printf(">>>> searching time: %ld milli seconds\n", (long)ms[2]);
     TIMER_START(a);
     print_ms(">>>> index destruction time", &a);
+}
+
+static void read_any(const char *fname, chunk_buf *c, int compressed)
+{
+    if (compressed)
+        read_compressed_chunk(fname, c);
+    else
+        read_chunk(fname, c);
 }
 
 int main(int argc, char *argv[])
@@ -92,7 +136,7 @@ int main(int argc, char *argv[])
     char err[4096];
     float *rad, *r2;
     uint64_t *counts;
-    chunk_buf bufs[2] = {{NULL, 0, 0, 0}, {NULL, 0, 0, 0}};
+    chunk_buf bufs[2];
     int nfiles, i, hthreads, device = 0, rc;
     const char *dev_env = getenv("SJCS_DEVICE");
 
@@ -103,11 +147,7 @@ int main(int argc, char *argv[])
         sjcs_cli_print_usage(stderr, argv[0]);
         exit(EXIT_FAILURE);
     }
-    if (cli.compressed) {
-        fprintf(stderr, "\n => --compressed (Gadget) particle input is not supported by this "
-                        "engine; decode the files first.\n");
-        exit(EXIT_FAILURE);
-    }
+    memset(bufs, 0, sizeof(bufs));
     hthreads = cli.nthreads < 256 ? cli.nthreads : 256;
     if (dev_env)
         device = atoi(dev_env);
@@ -148,18 +188,20 @@ int main(int argc, char *argv[])
     TIMER_START(tvtotal);
     puts(cli.maxbacklogs == 0 ? "  -- with no backlog execution" : "  -- with backlog execution");
     fflush(stdout);
-    read_chunk(argv[cli.particle_beg], &bufs[0]);
+    read_any(argv[cli.particle_beg], &bufs[0], cli.compressed);
     for (i = 0; i < nfiles; i++) {
         chunk_buf *cur = &bufs[i & 1];
         printf(">>>> particles reading time: %ld milli seconds\n", cur->read_ms);
         if (cur->n == 0)
             fprintf(stderr, "%s(): No data loaded. Cannot make tree.\n",
                     "MakeTreeWithSameThreadPoolArraySTR");
-        if (sjcs_add_particles(e, cur->xyz, cur->n) != SJCS_OK)
+        rc = cli.compressed ? sjcs_add_compressed_particles(e, cur->uv, cur->n, &cur->dec)
+                            : sjcs_add_particles(e, cur->xyz, cur->n);
+        if (rc != SJCS_OK)
             die_engine(e, "particle chunk");
         if (i + 1 < nfiles)
-            read_chunk(argv[cli.particle_beg + i + 1], &bufs[(i + 1) & 1]);
-        finish_chunk(e, cur->n);
+            read_any(argv[cli.particle_beg + i + 1], &bufs[(i + 1) & 1], cli.compressed);
+        finish_chunk(e, cur->n, cli.compressed);
         fflush(stdout);
     }
     print_ms(">>>> total searching time", &tvtotal);
@@ -174,8 +216,10 @@ int main(int argc, char *argv[])
         exit(EXIT_FAILURE);
     print_ms(">>>> result writing time", &tv);
 
-    free(bufs[0].xyz);
-    free(bufs[1].xyz);
+    for (i = 0; i < 2; i++) {
+        free(bufs[i].xyz);
+        free(bufs[i].uv);
+    }
     sjcs_free_halos(&halos);
     free(counts);
     free(rad);

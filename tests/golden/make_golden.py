@@ -20,6 +20,7 @@ import os
 import re
 import shutil
 import subprocess
+import struct
 import sys
 import tempfile
 
@@ -50,6 +51,64 @@ def write_halos(path, pts, ids=None, fmt="%f", trailing_newline=True, raw_lines=
 
 def write_particles(path, pts):
     np.ascontiguousarray(pts, dtype=np.float32).tofile(path)
+
+
+def write_compressed(path, uv, npart_total, box=1000.0, disp_min=-30.0, disp_max=30.0,
+                     pos_bits=16, id_start=1):
+    """A compressed (Gadget) particle file as readCompressedParticles reads it
+    (src/Gadget/BufVector3D.c:111-136): record marker, 256-byte header
+    (include/Gadget/GadgetHeader.h:4-27), two markers, uint16 triplets,
+    marker."""
+    uv = np.ascontiguousarray(uv, dtype="<u2").reshape(-1, 3)
+    n = uv.shape[0]
+    hdr = bytearray(256)
+    struct.pack_into("<6i", hdr, 0, 0, n, 0, 0, 0, 0)                    # npart
+    lo = npart_total & 0xFFFFFFFF
+    lo = lo - (1 << 32) if lo >= (1 << 31) else lo
+    struct.pack_into("<6i", hdr, 96, 0, lo, npart_total >> 32, 0, 0, 0)  # npartTotal
+    struct.pack_into("<i", hdr, 124, 1)                                  # num_files
+    struct.pack_into("<d", hdr, 128, box)                                # BoxSize
+    struct.pack_into("<ffi", hdr, 172, disp_min, disp_max, pos_bits)
+    struct.pack_into("<qq", hdr, 184, id_start, id_start + n - 1)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", 256) + bytes(hdr) + struct.pack("<i", 256))
+        f.write(struct.pack("<i", 6 * n) + uv.tobytes() + struct.pack("<i", 6 * n))
+
+
+def fma_sensitive_halos(path_particles, r2, shell, limit=40):
+    """Halo centres that tell the two decodes apart: for particles whose
+    coordinate on exactly one axis differs between cell_interval*ip +
+    disp_min fused (one FMA) and unfused (two roundings), a centre on the
+    particle's other two coordinates whose squared distance along that axis
+    falls in `shell` for one decode and not for the other."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import sjcs_testlib as tl
+    fused = tl.oracle_decode_file(path_particles, fused=1)
+    plain = tl.oracle_decode_file(path_particles, fused=0)
+    diff = fused != plain
+    one = np.nonzero(diff.sum(axis=1) == 1)[0]
+    r = np.float32(np.sqrt(np.float64(r2[shell])))
+    out = []
+    for i in one:
+        d = int(np.nonzero(diff[i])[0][0])
+        base = np.float32(fused[i, d] - r)
+        cands = [base]
+        up = dn = base
+        for _ in range(64):
+            up = np.nextafter(up, np.float32(np.inf))
+            dn = np.nextafter(dn, np.float32(-np.inf))
+            cands += [up, dn]
+        for cd in cands:
+            df = np.float32(fused[i, d] - cd)
+            dp = np.float32(plain[i, d] - cd)
+            if tl.shell_index(np.float32(df * df), r2) != tl.shell_index(np.float32(dp * dp), r2):
+                c = fused[i].copy()
+                c[d] = cd
+                out.append(c)
+                break
+        if len(out) >= limit:
+            break
+    return np.array(out, np.float32)
 
 
 def write_gz(src, dst):
@@ -186,6 +245,61 @@ def main():
         f.write("\n".join(str(i) for i in ids) + "\n")
     case("uni_filter", ["uni_particles.dat"], "uni_halos.list", "0.001:5:8",
          filter_file="uni_filter.txt")
+
+    # 9. compressed (Gadget) particle files (--compressed).  A generator of
+    #    its own, so the fixtures above stay as they were.
+    crng = np.random.default_rng(20261204)
+    cz = ["--compressed"]
+
+    def uv_random(n, bits=16):
+        return crng.integers(0, 1 << bits, (n, 3), dtype=np.uint16)
+
+    # a whole 24^3 grid, spacing 1000/24 (inexact in FP32), displacements
+    # +-30: particles below 0 and near 1000 on every axis
+    write_compressed(os.path.join(GOLD, "gad_grid.gdt"), uv_random(24 ** 3), 24 ** 3)
+    case("gad_grid", ["gad_grid.gdt"], "uni_halos.list", "0.5:20:12", extra=cz)
+    # one 32^3 snapshot in two files with their own displacement encodings
+    uv2 = uv_random(32 ** 3, 12)
+    write_compressed(os.path.join(GOLD, "gad_part0.gdt"), uv2[:16384], 32 ** 3,
+                     disp_min=-20.0, disp_max=20.0, pos_bits=12, id_start=1)
+    write_compressed(os.path.join(GOLD, "gad_part1.gdt"), uv2[16384:], 32 ** 3,
+                     disp_min=-12.5, disp_max=7.75, pos_bits=12, id_start=16385)
+    case("gad_2files", ["gad_part0.gdt", "gad_part1.gdt"], "uni_halos.list", "0.5:30:10",
+         extra=cz)
+    # edges: id 0 (grid coordinate -1), ids past the grid (a second tile:
+    # x beyond the box), extreme triplets; a half-size box with a negative
+    # displacement span and 10-bit triplets
+    uve = uv_random(8200)
+    uve[:50] = 0
+    uve[50:100] = 65535
+    write_compressed(os.path.join(GOLD, "gad_edge0.gdt"), uve, 20 ** 3, disp_min=-50.0,
+                     disp_max=50.0, id_start=0)
+    write_compressed(os.path.join(GOLD, "gad_edge1.gdt"), uv_random(1000, 10), 10 ** 3,
+                     box=500.0, disp_min=5.0, disp_max=-5.0, pos_bits=10, id_start=1)
+    case("gad_edge", ["gad_edge0.gdt", "gad_edge1.gdt"], "uni_halos.list", "1:25:6", extra=cz)
+    # a 1700^3 snapshot (npartTotal split over two words), one file of 6000
+    # particles deep inside: 64-bit ids, inexact spacing; halos near them
+    ng = 1700
+    idb = 1 + ng * ng * 900 + ng * 850 + 100
+    write_compressed(os.path.join(GOLD, "gad_big.gdt"), uv_random(6000), ng ** 3,
+                     disp_min=-2.0, disp_max=2.0, id_start=idb)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import sjcs_testlib as tl
+    pb = tl.oracle_decode_file(os.path.join(GOLD, "gad_big.gdt"), fused=1)
+    hb = pb[crng.integers(0, pb.shape[0], 300)] + crng.uniform(-3, 3, (300, 3)).astype(np.float32)
+    hb = np.vstack([hb, crng.random((100, 3)) * 1000.0])
+    write_halos(os.path.join(GOLD, "gad_big_halos.list"), hb, fmt="%.9g")
+    case("gad_big", ["gad_big.gdt"], "gad_big_halos.list", "0.05:3:10", extra=cz)
+    # halos placed where the fused and the unfused decode disagree about a
+    # shell: this case tells which of the two the reference computes
+    write_compressed(os.path.join(GOLD, "gad_fma.gdt"), uv_random(48 * 48 * 2), 48 ** 3,
+                     disp_min=-7.3, disp_max=7.3, id_start=1 + 48 * 48 * 20)
+    _, r2f = tl.oracle_thresholds(0.5, 4.0, 8)
+    hf = fma_sensitive_halos(os.path.join(GOLD, "gad_fma.gdt"), r2f, 5)
+    if hf.shape[0] < 10:
+        raise RuntimeError(f"gad_fma: only {hf.shape[0]} sensitive halos")
+    write_halos(os.path.join(GOLD, "gad_fma_halos.list"), hf, fmt="%.9g")
+    case("gad_fma", ["gad_fma.gdt"], "gad_fma_halos.list", "0.5:4:8", extra=cz)
 
     # 8. option errors: stderr and exit status (argv[0] shown as ARGV0)
     errors = []

@@ -63,6 +63,15 @@ def oracle():
     lib.oracle_images.argtypes = [vp, ctypes.c_float, vp]
     lib.oracle_box_radius.restype = ctypes.c_float
     lib.oracle_box_radius.argtypes = [vp, ctypes.c_int]
+    ll = ctypes.c_longlong
+    lib.oracle_gadget_params.restype = None
+    lib.oracle_gadget_params.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+    lib.oracle_gadget_gridpos.restype = None
+    lib.oracle_gadget_gridpos.argtypes = [ll, ll, ctypes.c_int, vp]
+    lib.oracle_gadget_decode.restype = None
+    lib.oracle_gadget_decode.argtypes = [vp, ctypes.c_uint64, ll, ll, ctypes.c_int,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                         ctypes.c_int, vp]
     _oracle = lib
     return lib
 
@@ -83,6 +92,51 @@ def oracle_count(particles, halos, r2, threads=8):
     w = oracle().oracle_count(p.ctypes.data, p.shape[0], h.ctypes.data, 12, h.shape[0],
                               r2.ctypes.data, r2.size, c.ctypes.data, threads)
     return c, w
+
+
+def shell_index(dist, r2):
+    """First shell k with dist < r2[k], or -1 (MultiRangeCount.c:107-113)."""
+    r2 = np.ascontiguousarray(r2, np.float32)
+    return oracle().oracle_shell_of(float(dist), r2.ctypes.data, r2.size)
+
+
+def gadget_params(hdr, ntile=1):
+    """The oracle's decode parameters of a 256-byte header, as a dict."""
+    lib = oracle()
+    h = np.frombuffer(bytes(hdr), np.uint8).copy()
+    ids, ng = ctypes.c_longlong(), ctypes.c_longlong()
+    nt = ctypes.c_int()
+    ci, dmin, us = ctypes.c_float(), ctypes.c_float(), ctypes.c_double()
+    lib.oracle_gadget_params(h.ctypes.data, ntile, ctypes.byref(ids), ctypes.byref(ng),
+                             ctypes.byref(nt), ctypes.byref(ci), ctypes.byref(dmin),
+                             ctypes.byref(us))
+    return {"id_start": ids.value, "ng1d": ng.value, "nt1d": nt.value,
+            "cell_interval": ci.value, "disp_min": dmin.value, "unitsep": us.value}
+
+
+def oracle_gadget_decode(uv, prm, fused=1):
+    uv = np.ascontiguousarray(uv, np.uint16).reshape(-1, 3)
+    out = np.zeros(uv.shape, np.float32)
+    oracle().oracle_gadget_decode(uv.ctypes.data, uv.shape[0], prm["id_start"], prm["ng1d"],
+                                  prm["nt1d"], prm["cell_interval"], prm["disp_min"],
+                                  prm["unitsep"], fused, out.ctypes.data)
+    return out
+
+
+def read_compressed(path):
+    """(256 header bytes, uint16 triplets [n][3]) of a compressed particle
+    file: skip 4, header, skip 8, npart[1] triplets (BufVector3D.c:111-136)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    hdr = raw[4:260]
+    n = int(np.frombuffer(hdr[4:8], "<i4")[0])
+    uv = np.frombuffer(raw[268:268 + 6 * n], "<u2").reshape(n, 3)
+    return hdr, uv
+
+
+def oracle_decode_file(path, fused=1):
+    hdr, uv = read_compressed(path)
+    return oracle_gadget_decode(uv, gadget_params(hdr), fused)
 
 
 def parse_radius(spec):
@@ -220,7 +274,10 @@ def filter_halos_nfilters_zero(ids, pts, filter_path):
 
 def case_inputs(case, filter_as_reference_runs=False):
     """Particle chunks, halo ids and points of a golden case."""
-    chunks = [read_particles(os.path.join(GOLD, p)) for p in case["particle_files"]]
+    if "--compressed" in case.get("extra", []):
+        chunks = [oracle_decode_file(os.path.join(GOLD, p)) for p in case["particle_files"]]
+    else:
+        chunks = [read_particles(os.path.join(GOLD, p)) for p in case["particle_files"]]
     ids, pts = read_halos(os.path.join(GOLD, case["halo_file"]))
     if case.get("filter_file"):
         fn = filter_halos_nfilters_zero if filter_as_reference_runs else filter_halos

@@ -515,6 +515,135 @@ def test_index_api_matches_oracle(built, eng):
 
 # ------------------------------------------------ full size, by properties --
 
+# ----------------------------------------------- compressed (Gadget) input --
+
+def gadget_params(built, total, box, dmin, dmax, bits, id_start, ntile=1, fused=1):
+    from test_oracle import gadget_header
+    prm = built.gadget_decode_params(gadget_header(total, box, dmin, dmax, bits, id_start),
+                                     ntile)
+    prm.fused = fused
+    return prm
+
+
+def as_dict(prm):
+    return {f: getattr(prm, f) for f in ("id_start", "ng1d", "nt1d", "cell_interval",
+                                         "disp_min", "unitsep")}
+
+
+DECODE_SETS = {
+    # whole grid and ids past it (a second tile along x)
+    "grid24": (24 ** 3, 1000.0, -30.0, 30.0, 16, 1, 1, 2_000_000),
+    # 2048^3 (total in npartTotal[2]), ids across the grid's end
+    "big2048": (2048 ** 3, 1000.0, -1.5, 2.5, 14, 2 ** 33 - 500_000, 1, 1_000_000),
+    # the largest side, ids just below 2^50
+    "side46340": (46340 ** 3, 1000.0, -0.25, 0.25, 16, 2 ** 50 - 1_500_000, 1, 1_000_000),
+    # ids below 1: C's truncating division of negative numbers
+    "negative": (100 ** 3, 1000.0, -5.0, 5.0, 16, -700_000, 1, 1_500_000),
+    # three tiles per side
+    "tiles3": (300 ** 3, 900.0, -3.0, 1.0, 12, 1, 3, 3_000_000),
+    # spacing inexact in FP32, fused and unfused differ
+    "fma48": (48 ** 3, 1000.0, -7.3, 7.3, 16, 1, 1, 110_592),
+}
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("name", sorted(DECODE_SETS))
+def test_decode_bit_exact(built, eng, name, fused):
+    total, box, dmin, dmax, bits, id0, ntile, n = DECODE_SETS[name]
+    prm = gadget_params(built, total, box, dmin, dmax, bits, id0, ntile, fused)
+    rng = np.random.default_rng(len(name) + fused)
+    uv = rng.integers(0, 1 << bits, (n, 3), dtype=np.uint16)
+    uv[:7] = [0, 0, 0]
+    uv[7:14] = (1 << bits) - 1
+    print("decode", name, "fused", fused, "n", n, flush=True)
+    got = eng.decode_compressed(uv, prm)
+    want = t.oracle_gadget_decode(uv, as_dict(prm), fused)
+    bad = np.nonzero((got.view(np.uint32) != want.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} particles differ, first {bad[:4]}: {got[bad[0]]} {want[bad[0]]}"
+    if name == "fma48":
+        other = t.oracle_gadget_decode(uv, as_dict(prm), 1 - fused)
+        assert not np.array_equal(got, other)
+
+
+def test_decode_degenerate_sizes(built, eng):
+    prm = gadget_params(built, 24 ** 3, 1000.0, -30.0, 30.0, 16, 5)
+    assert eng.decode_compressed(np.zeros((0, 3), np.uint16), prm).shape == (0, 3)
+    uv = np.array([[1, 2, 65535]], np.uint16)
+    assert np.array_equal(eng.decode_compressed(uv, prm), t.oracle_gadget_decode(uv, as_dict(prm)))
+
+
+def test_decode_rejects_what_it_cannot_decode_exactly(built, eng):
+    uv = np.zeros((4, 3), np.uint16)
+    cases = [("ng1d", 0), ("ng1d", 46341), ("nt1d", 0), ("nt1d", 46341),
+             ("id_start", 2 ** 50), ("id_start", 1 - 2 ** 50)]
+    for field, value in cases:
+        prm = gadget_params(built, 24 ** 3, 1000.0, -1.0, 1.0, 16, 1)
+        setattr(prm, field, value)
+        with pytest.raises(built.SJCSError) as ei:
+            eng.decode_compressed(uv, prm)
+        assert ei.value.code == built.SJCS_ERR_INVALID, (field, value)
+    # grid points past 2^31: side 10, ids near 3e11 (tile number 3e8)
+    prm = gadget_params(built, 10 ** 3, 1000.0, -1.0, 1.0, 16, 3 * 10 ** 11)
+    with pytest.raises(built.SJCSError) as ei:
+        eng.decode_compressed(uv, prm)
+    assert ei.value.code == built.SJCS_ERR_INVALID
+    # the engine stays usable, and a chunk it refuses adds nothing
+    _, r2 = t.oracle_thresholds(0.5, 30.0, 10)
+    eng.set_shells(r2)
+    eng.set_halos(np.array([[500.0, 500.0, 500.0]], np.float32))
+    with pytest.raises(built.SJCSError):
+        eng.add_compressed_particles(uv, prm)
+    assert int(eng.counts().sum()) == 0
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_compressed_files_equal_oracle(built, eng, graph):
+    """The golden two-file snapshot through the engine's compressed path:
+    counts equal the oracle's on its own decode."""
+    _, pts = t.read_halos(os.path.join(t.GOLD, "uni_halos.list"))
+    _, r2 = t.oracle_thresholds(0.5, 30.0, 10)
+    files = [os.path.join(t.GOLD, f) for f in ("gad_part0.gdt", "gad_part1.gdt")]
+    eng.set_option("graph", graph)
+    try:
+        eng.set_shells(r2)
+        eng.set_halos(pts)
+        for f in files:
+            hdr, uv = built.read_compressed(f)
+            eng.add_compressed_particles(uv, built.gadget_decode_params(hdr))
+        got = eng.counts()
+    finally:
+        eng.set_option("graph", 0)
+    want, _ = oracle_chunks([t.oracle_decode_file(f) for f in files], pts, r2)
+    assert_counts_equal(got, want, pts)
+
+
+def test_compressed_chunks_equal_decoded_chunks(built, eng):
+    """Chunks of growing and shrinking size through both slots: the
+    compressed path counts exactly what the decoded path counts for the
+    oracle's decode of the same triplets."""
+    rng = np.random.default_rng(77)
+    ng = 110
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 8)
+    h = rng.random((20_000, 3), dtype=np.float32) * np.float32(1000.0)
+    chunks = []
+    start = 1
+    for n in (400_000, 1_100_000, 250_000, 600_000):
+        prm = gadget_params(built, ng ** 3, 1000.0, -4.0, 4.0, 16, start)
+        chunks.append((rng.integers(0, 65536, (n, 3), dtype=np.uint16), prm))
+        start += n
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    for uv, prm in chunks:
+        eng.add_compressed_particles(uv, prm)
+    got = eng.counts()
+    eng.reset_counts()
+    for uv, prm in chunks:
+        eng.add_particles(t.oracle_gadget_decode(uv, as_dict(prm)))
+    want = eng.counts()
+    assert_counts_equal(got, want, h)
+    assert int(got.sum()) > 0
+
+
 def test_full_size_config2_properties(eng):
     """Config 2 at its full size (1e8 particles x 1e7 halos x 40 shells),
     where the oracle is too slow: counts are independent of the cell size

@@ -180,3 +180,79 @@ def test_cli_missing_files_fail_cleanly(built, tmp_path):
                        timeout=60)
     assert r.returncode == 1
     assert "ERROR: file nope.list not found in readHalos()." in r.stderr
+
+
+# ---------------------------------------------- compressed (Gadget) files --
+
+GADGET_FILES = sorted({p for c in CASES if "--compressed" in c["extra"]
+                       for p in c["particle_files"]})
+
+
+def host_read_compressed(built, path):
+    lib = ctypes.CDLL(built.HOST_LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.sjcs_read_compressed.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(vp),
+                                         ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint64)]
+    hdr = ctypes.create_string_buffer(256)
+    uv = vp()
+    cap, n = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = lib.sjcs_read_compressed(path.encode(), hdr, ctypes.byref(uv), ctypes.byref(cap),
+                                  ctypes.byref(n))
+    out = None
+    if rc == 0 and n.value:
+        out = np.ctypeslib.as_array(ctypes.cast(uv, ctypes.POINTER(ctypes.c_uint16)),
+                                    (n.value * 3,)).reshape(-1, 3).copy()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.free.argtypes = [vp]
+    libc.free(uv)
+    return rc, hdr.raw, out
+
+
+def test_compressed_reader(built):
+    assert len(GADGET_FILES) >= 6
+    for name in GADGET_FILES:
+        path = os.path.join(t.GOLD, name)
+        rc, hdr, uv = host_read_compressed(built, path)
+        want_hdr, want_uv = t.read_compressed(path)
+        assert rc == 0 and hdr == want_hdr
+        assert np.array_equal(uv, want_uv)
+        h2, uv2 = built.read_compressed(path)
+        assert h2 == want_hdr and np.array_equal(uv2, want_uv)
+
+
+def test_compressed_reader_errors(built, tmp_path):
+    good = open(os.path.join(t.GOLD, "gad_grid.gdt"), "rb").read()
+    assert host_read_compressed(built, str(tmp_path / "none"))[0] == -1
+    short = tmp_path / "short.gdt"
+    short.write_bytes(good[:-100])
+    assert host_read_compressed(built, str(short))[0] == -2
+    neg = bytearray(good)
+    neg[8:12] = (-5).to_bytes(4, "little", signed=True)
+    bad = tmp_path / "neg.gdt"
+    bad.write_bytes(bytes(neg))
+    assert host_read_compressed(built, str(bad))[0] == -2
+    empty = bytearray(good[:268])
+    empty[8:12] = (0).to_bytes(4, "little")
+    e = tmp_path / "empty.gdt"
+    e.write_bytes(bytes(empty))
+    rc, _, uv = host_read_compressed(built, str(e))
+    assert rc == 0 and uv is None
+
+
+def test_gadget_decode_params_equal_oracle(built):
+    """The library's header -> decode parameters (host code, no device)
+    equal the oracle's on every golden file and on ntile 2."""
+    from test_oracle import gadget_header
+    headers = [t.read_compressed(os.path.join(t.GOLD, n))[0] for n in GADGET_FILES]
+    headers += [gadget_header(2048 ** 3, 1000.0, -1.5, 2.5, 14, 3 * 10 ** 9),
+                gadget_header(1700 ** 3, 750.0, 3.0, -3.0, 8, -7)]
+    for hdr in headers:
+        for ntile in (1, 2):
+            got = built.gadget_decode_params(hdr, ntile)
+            want = t.gadget_params(hdr, ntile)
+            assert got.fused == 1
+            for f in ("id_start", "ng1d", "nt1d", "cell_interval", "disp_min", "unitsep"):
+                assert getattr(got, f) == want[f], f
+    with pytest.raises(built.SJCSError):
+        built.gadget_decode_params(gadget_header(1625 ** 3, 1000.0, 0.0, 1.0, 16, 1))

@@ -143,3 +143,151 @@ def test_images_order_and_shift_rule():
                         np.float32(600.0), cent.ctypes.data)
     assert n == 8 and cent[:n].max(axis=0).tolist() == [1500.0, 1500.0, 1500.0]
     assert cent[:n].min(axis=0).tolist() == [500.0, 500.0, 500.0]
+
+
+# ------------------------------------------------- compressed (Gadget) decode --
+
+from fractions import Fraction  # noqa: E402
+
+
+def f32_nearest(x: Fraction) -> np.float32:
+    """x rounded to the nearest FP32 value, ties to even, from exact x."""
+    if x == 0:
+        return np.float32(0.0)
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    ulp = Fraction(2) ** max(e - 23, -149)
+    q = a / ulp
+    n = q.numerator // q.denominator
+    rest = q - n
+    if rest > Fraction(1, 2) or (rest == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return np.float32(float(n * ulp) if x > 0 else -float(n * ulp))
+
+
+def lattice_point(pid, ng):
+    """Closed form of a grid id 1..ng^3 with one tile: id - 1 counts the
+    lattice with z fastest, then y, then x."""
+    k = pid - 1
+    return k // (ng * ng), (k // ng) % ng, k % ng
+
+
+def exact_decode(pid, uv, ng, ci, dmin, unitsep, fused):
+    """pos[d] = (float)((double)s + unitsep*uv[d]), s = ci*ip + dmin rounded
+    once (fused) or twice, each rounding from the exact value."""
+    out = []
+    for d, ip in enumerate(lattice_point(pid, ng)):
+        prod = Fraction(float(ci)) * ip
+        if fused:
+            s = f32_nearest(prod + Fraction(float(dmin)))
+        else:
+            s = f32_nearest(Fraction(float(f32_nearest(prod))) + Fraction(float(dmin)))
+        v = float(Fraction(float(s)) + Fraction(unitsep) * int(uv[d]))
+        out.append(np.float32(v))
+    return out
+
+
+def gadget_header(total, box, dmin, dmax, bits, id_start, npart=0):
+    import struct
+    h = bytearray(256)
+    struct.pack_into("<6i", h, 0, 0, npart, 0, 0, 0, 0)
+    lo = total & 0xFFFFFFFF
+    lo = lo - (1 << 32) if lo >= (1 << 31) else lo
+    struct.pack_into("<6i", h, 96, 0, lo, total >> 32, 0, 0, 0)
+    struct.pack_into("<d", h, 128, box)
+    struct.pack_into("<ffi", h, 172, dmin, dmax, bits)
+    struct.pack_into("<q", h, 184, id_start)
+    return bytes(h)
+
+
+@pytest.mark.parametrize("ng,box,dmin,dmax,bits", [
+    (24, 1000.0, -30.0, 30.0, 16), (48, 1000.0, -7.3, 7.3, 16), (1700, 1000.0, -2.0, 2.0, 16),
+    (37, 687.5, -11.1, 3.3, 12), (10, 500.0, 5.0, -5.0, 10)])
+def test_gadget_decode_exact_arithmetic(ng, box, dmin, dmax, bits):
+    """The oracle's decode equals the decode computed from exact rationals
+    with each of the reference's roundings applied once, fused and not, on
+    ids spread over the whole grid; the two differ somewhere, so the check
+    tells them apart."""
+    rng = np.random.default_rng(ng)
+    prm = t.gadget_params(gadget_header(ng ** 3, box, dmin, dmax, bits, 1))
+    assert prm["ng1d"] == ng and prm["nt1d"] == 1
+    assert prm["cell_interval"] == np.float32(box / ng)
+    assert prm["unitsep"] == float(np.float32(np.float32(dmax) - np.float32(dmin))) / 2 ** bits
+    ids = rng.integers(1, ng ** 3 + 1, 600)
+    uv = rng.integers(0, 1 << bits, (600, 3), dtype=np.uint16)
+    seen_diff = False
+    for fused in (1, 0):
+        got = np.stack([t.oracle_gadget_decode(uv[i], dict(prm, id_start=int(ids[i])), fused)[0]
+                        for i in range(600)])
+        want = np.array([exact_decode(int(ids[i]), uv[i], ng, prm["cell_interval"],
+                                      prm["disp_min"], prm["unitsep"], fused)
+                         for i in range(600)], np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), fused
+        if fused:
+            other = want
+        else:
+            seen_diff = not np.array_equal(other, want)
+    if ng in (48, 1700, 37):  # grids where some ci*ip + dmin rounds differently
+        assert seen_diff
+
+
+def test_gadget_grid_closed_form():
+    """Grid points of ids 1..ng^3 (one tile) are the lattice, z fastest; a
+    second tile (ids past ng^3) continues along x; id 0 is (0, 0, -1)."""
+    o = t.oracle()
+    ip = np.zeros(3, np.int32)
+    ng = 7
+    pts = set()
+    for pid in range(1, ng ** 3 + 1):
+        o.oracle_gadget_gridpos(pid, ng, 1, ip.ctypes.data)
+        assert tuple(ip) == lattice_point(pid, ng)
+        pts.add(tuple(ip))
+    assert len(pts) == ng ** 3
+    o.oracle_gadget_gridpos(ng ** 3 + 1, ng, 1, ip.ctypes.data)
+    assert tuple(ip) == (ng, 0, 0)
+    o.oracle_gadget_gridpos(0, ng, 1, ip.ctypes.data)
+    assert tuple(ip) == (0, 0, -1)
+    # two tiles per side: tile t = (id-1) // ng^3 has digits (tx, ty, tz)
+    # base 2, and shifts the lattice by ng per digit
+    for pid in range(1, 8 * ng ** 3 + 1, 13):
+        o.oracle_gadget_gridpos(pid, ng, 2, ip.ctypes.data)
+        tile, inner = divmod(pid - 1, ng ** 3)
+        g = lattice_point(inner + 1, ng)
+        tdig = (tile // 4, (tile // 2) % 2, tile % 2)
+        assert tuple(ip) == tuple(tdig[d] * ng + g[d] for d in range(3))
+
+
+def test_gadget_params_totals():
+    """nparticles_at1d is the cube root of npartTotal[1] + npartTotal[2]<<32
+    (whole for exact cubes, also past 2^32), ng1d = at1d / ntile."""
+    for b in (1, 2, 24, 1000, 1200, 1700, 2048, 3000):
+        prm = t.gadget_params(gadget_header(b ** 3, 1000.0, 0.0, 1.0, 16, 1))
+        assert prm["ng1d"] == b
+        prm = t.gadget_params(gadget_header(b ** 3, 1000.0, 0.0, 1.0, 16, 1), ntile=2)
+        assert prm["ng1d"] == b // 2 and prm["nt1d"] == 2
+    prm = t.gadget_params(gadget_header(25 ** 3 - 1, 1000.0, 0.0, 1.0, 16, 1))
+    assert prm["ng1d"] == 24  # rounds down
+    # npartTotal[1] is an int there: totals in [2^31, 2^32) read as negative
+    # and give no grid (the product refuses such a header)
+    prm = t.gadget_params(gadget_header(1625 ** 3, 1000.0, 0.0, 1.0, 16, 1))
+    assert prm["ng1d"] <= 0
+
+
+def test_gadget_fma_case_tells_the_decodes_apart():
+    """gad_fma's halos sit where the fused and the unfused decode put a
+    particle in different shells: the reference's golden result equals the
+    fused decode's counts (checked with every golden case) and differs from
+    the unfused decode's in every row."""
+    case = [c for c in CASES if c["name"] == "gad_fma"][0]
+    mn, mx, k = t.parse_radius(case["radius"])
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    _, pts = t.read_halos(t.GOLD + "/" + case["halo_file"])
+    _, ecnt, _, _ = t.parse_result(t.read_expected_text(case["expected"]), k)
+    plain = t.oracle_decode_file(t.GOLD + "/" + case["particle_files"][0], fused=0)
+    c0, _ = t.oracle_count(plain, pts, r2)
+    assert pts.shape[0] >= 10
+    assert (c0 != ecnt).any(axis=1).all()
