@@ -123,7 +123,7 @@ def run_ref(args, cwd, binary=REF):
 
 
 def mask_stdout(s):
-    return re.sub(r"(>>>> [a-z ]+ time: )\d+( milli seconds)", r"\1#\2", s)
+    return re.sub(r"(>>>> [a-z +]+ time: )\d+( milli seconds)", r"\1#\2", s)
 
 
 def main():

@@ -69,7 +69,7 @@ def assert_counts_equal(got, want, halos):
 # ------------------------------------------------------------- golden CLI --
 
 def mask(s):
-    return re.sub(r"(>>>> [a-z ]+ time: )\d+( milli seconds)", r"\1#\2", s)
+    return re.sub(r"(>>>> [a-z +]+ time: )\d+( milli seconds)", r"\1#\2", s)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
