@@ -12,6 +12,10 @@ file).  Inputs are in HBM before the timed region starts; the halos are
 sorted once beforehand, as the reference reads its halo file once before its
 chunk loop.
 
+Config c2z is the same search with the particles given as a compressed
+(Gadget) snapshot: a step then starts from the device-resident uint16
+triplets and decodes them on the device before the sort.
+
 ``value`` = W / (seconds per step), W = exact number of in-box
 (halo image, particle) pair tests of the chunk -- the reference's
 CalcDistance calls -- counted by an untimed pass.
@@ -63,6 +67,10 @@ CONFIGS = {
     "c3": dict(workload="uniform 3 x 1e8 particles (3 files) x 1e7 halos, --radius 0.001:5:40, "
                         "u64 counts accumulated on the device",
                np=100_000_000, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=3, graph=True),
+    "c2z": dict(workload="compressed Gadget snapshot: 464^3 grid (99.9M particles, uint16 "
+                         "displacements over +-20) x 1e7 uniform halos, --radius 0.001:5:40, "
+                         "decoded on the device",
+                np=464 ** 3, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=1, compressed=464),
 }
 
 
@@ -72,6 +80,7 @@ METRICS = {
     "c3": "shell pair-tests/s (10M halos x 3 x 100M particles x 40 shells, 1 GPU)",
     "c4": "shell pair-tests/s (5M clustered halos x 200M clustered particles x 64 shells, 1 GPU)",
     "c5": "served requests/s (3e8-particle resident index, batches of 1-1e5 halos, own shells)",
+    "c2z": "shell pair-tests/s (10M halos x 99.9M compressed particles x 40 shells, 1 GPU)",
 }
 
 
@@ -82,6 +91,9 @@ def log(msg):
 def make_inputs(cfg, seed):
     """Particle files (one array each, distinct seeds) and the halos."""
     rng = np.random.default_rng(seed)
+    if cfg.get("compressed"):
+        part = synth.gadget_snapshot(rng, cfg["compressed"])
+        return [part], synth.uniform_points(rng, cfg["nh"])
     if cfg.get("clustered"):
         parts, halos = synth.clustered(rng, cfg["np"], cfg["nh"])
         return [parts], halos
@@ -111,6 +123,38 @@ def timer_lines(text):
             for m in re.finditer(r">>>> ([a-z ]+) time: (\d+) milli seconds", text)}
 
 
+def is_compressed(part):
+    return isinstance(part, tuple)
+
+
+def write_particle_files(tmp, parts):
+    """One file per chunk: packed FP32 xyz, or a compressed (Gadget) file for
+    a (header, triplets) chunk.  Returns the paths and the CLI flag."""
+    pfs = []
+    for c, p in enumerate(parts):
+        if is_compressed(p):
+            pfs.append(os.path.join(tmp, f"part{c}.gdt"))
+            sj.write_compressed(pfs[-1], *p)
+        else:
+            pfs.append(os.path.join(tmp, f"part{c}.dat"))
+            p.tofile(pfs[-1])
+    flag = ["--compressed"] if parts and is_compressed(parts[0]) else []
+    return pfs, flag
+
+
+def chunk_size(part):
+    return part[1].shape[0] if is_compressed(part) else part.shape[0]
+
+
+def add_device_chunk(eng, d, part):
+    """Queues one device-resident chunk: decoded points, or triplets decoded
+    on the device."""
+    if is_compressed(part):
+        eng.add_compressed_particles_device(d, part[1].shape[0], sj.gadget_decode_params(part[0]))
+    else:
+        eng.add_particles_device(d, part.shape[0])
+
+
 def wall_legs(parts, halos, radius, threads, with_reference):
     """End-to-end process wall time of the drop-in CLI (and, if asked, of the
     reference binary built from its sources) on files written here: reading
@@ -118,15 +162,13 @@ def wall_legs(parts, halos, radius, threads, with_reference):
     (to /dev/null: disk speed is not measured)."""
     tmp = tempfile.mkdtemp(prefix="sjcs_wall_")
     try:
-        pfs = []
-        for c, p in enumerate(parts):
-            pfs.append(os.path.join(tmp, f"part{c}.dat"))
-            p.tofile(pfs[-1])
+        pfs, flag = write_particle_files(tmp, parts)
         hf = os.path.join(tmp, "halo.list")
         write_halo_file(hf, halos)
         mn, mx, k = radius
         args = ["--thread", str(threads), "--next-setup", str(threads), "--particle-files", *pfs,
-                "--halo-file", hf, "--radius", f"{mn}:{mx}:{k}", "--output-file", os.devnull]
+                *flag, "--halo-file", hf, "--radius", f"{mn}:{mx}:{k}",
+                "--output-file", os.devnull]
         res = {}
         legs = [("sjcs_rcount", CLI_BIN)]
         if with_reference and os.path.exists(REF_BIN):
@@ -154,10 +196,7 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
         return None
     tmp = tempfile.mkdtemp(prefix="sjcs_cpu_")
     try:
-        pfs = []
-        for c, p in enumerate(parts):
-            pfs.append(os.path.join(tmp, f"part{c}.dat"))
-            p.tofile(pfs[-1])
+        pfs, flag = write_particle_files(tmp, parts)
         hf = os.path.join(tmp, "halo.list")
         # a random sample (the clustered set lists its clump-centre halos first)
         pick = np.sort(np.random.default_rng(7).choice(halos.shape[0], nsample, replace=False))
@@ -171,7 +210,7 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
             f.writelines(lines)
         mn, mx, k = radius
         cmd = [REF_BIN, "--thread", str(threads), "--next-setup", str(threads),
-               "--backlog", "0", "--particle-files", *pfs, "--halo-file", hf,
+               "--backlog", "0", "--particle-files", *pfs, *flag, "--halo-file", hf,
                "--radius", f"{mn}:{mx}:{k}", "--output-file", os.devnull]
         t0 = time.time()
         out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
@@ -185,10 +224,10 @@ def cpu_baseline(parts, halos, radius, eng, d_parts, nsample, threads):
         eng.set_halos(hs)
         w = 0
         for d, p in zip(d_parts, parts):
-            eng.add_particles_device(d, p.shape[0])
+            add_device_chunk(eng, d, p)
             w += eng.last_chunk_pair_tests()
         eng.set_halos(halos)
-        npart = sum(p.shape[0] for p in parts)
+        npart = sum(chunk_size(p) for p in parts)
         return {"value": w / search_s, "unit": "pair-tests/s", "cores": threads,
                 "kind": "reference",
                 "sample": f"astr_rcount --thread {threads}: all {npart} particles "
@@ -354,15 +393,17 @@ def main():
         eng.set_option("graph", 1)
     eng.set_shells(r2)
     eng.set_halos(halos)
-    d_parts = [eng.device_array(p) for p in parts]
+    d_parts = [eng.device_array(p[1] if is_compressed(p) else p) for p in parts]
     n = cfg["np"]
 
     def step():
         eng.reset_counts()
-        for d in d_parts:
-            eng.add_particles_device(d, n)
+        for d, p in zip(d_parts, parts):
+            add_device_chunk(eng, d, p)
 
     tuned = None
+    if args.autotune and cfg.get("compressed"):
+        sys.exit("bench.py: --autotune takes decoded particles")
     if args.autotune or (cfg.get("autotune") and not args.plain):
         t_at = time.perf_counter()
         best = eng.autotune(d_parts[0], n)
@@ -373,9 +414,9 @@ def main():
 
     # exact W of one step: an untimed pair-test pass per chunk
     w = 0
-    for d in d_parts:
+    for d, p in zip(d_parts, parts):
         eng.reset_counts()
-        eng.add_particles_device(d, n)
+        add_device_chunk(eng, d, p)
         w += eng.last_chunk_pair_tests()
 
     log(f"[bench] warmup {args.warmup}")
@@ -395,6 +436,7 @@ def main():
     ms_step = elapsed * 1000.0 / args.steps
 
     counted = int(eng.counts().sum())
+    upload_ms = float(phase[:, 0].mean())
     build_ms = float(phase[:, 1].mean())
     count_ms = float(phase[:, 2].mean())
     log(f"[bench] W={w} counted={counted} step={ms_step:.3f} ms build={build_ms:.3f} "
@@ -428,8 +470,22 @@ def main():
                 "traffic": traffic, "kernel": "cell sort (cs_* kernels)",
                 "bytes_per_launch": build_bytes}
 
+    decode = None
+    if cfg.get("compressed"):
+        # device decode: 6 B of triplets read, 12 B of FP32 xyz written per particle
+        decode_bytes = 18 * n
+        dec_gbs = decode_bytes / (upload_ms * 1e-3) / 1e9
+        decode = {"kernel": "gadget_decode_kernel", "ms": round(upload_ms, 4),
+                  "bytes_per_launch": decode_bytes, "achieved": round(dec_gbs, 1),
+                  "unit": "GB/s", "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
+                  "traffic": None}
+        tfile = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                decode["traffic"] = json.load(f).get(args.config, {}).get("decode")
+
     cpu = None
-    if not args.no_cpu_baseline and args.config in ("c2", "c3", "c4"):
+    if not args.no_cpu_baseline and args.config in ("c2", "c3", "c4", "c2z"):
         log("[bench] cpu baseline (reference astr_rcount on a halo sample)")
         nsample = args.cpu_halos if args.config != "c4" else min(args.cpu_halos, nh // 100)
         cpu = cpu_baseline(parts, halos, cfg["radius"], eng, d_parts,
@@ -461,6 +517,7 @@ def main():
                    "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
                    "pairs_counted": counted, "parallelism": "single GPU"},
         "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4),
+                      **({"decode": round(upload_ms, 4)} if cfg.get("compressed") else {}),
                       "of": ("the last chunk of a step" if cfg["chunks"] > 1 else "the chunk") +
                             (" (graph replay: sort and count timed together as count)"
                              if args.graph or cfg.get("graph") else "")},
@@ -469,6 +526,7 @@ def main():
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
                          "hbm_gbs_algorithmic": round(count_gbs, 1)},
         "autotune": tuned,
+        "decode_kernel": decode,
         "wall": wall,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -151,6 +151,10 @@ int sjcs_gadget_decode_params(const sjcs_gadget_header *hdr, int ntile,
  * handles exactly (grid points beyond +-2^31, |id| >= 2^50). */
 int sjcs_add_compressed_particles(sjcs_engine *e, const uint16_t *uv, uint64_t n,
                                   const sjcs_gadget_decode *p);
+/* The same with the triplets already on the device (d_uv, 3n uint16); the
+ * upload phase of sjcs_last_chunk_timings is then the decode alone. */
+int sjcs_add_compressed_particles_device(sjcs_engine *e, const uint16_t *d_uv, uint64_t n,
+                                         const sjcs_gadget_decode *p);
 
 /* Decodes n triplets at d_uv (device) into packed xyz at d_xyz (device);
  * returns when done.  The decode alone, for callers that keep the points. */

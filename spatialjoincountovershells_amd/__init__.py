@@ -46,6 +46,7 @@ EXPORTED = (
     "sjcs_index_count", "sjcs_index_destroy",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
     "sjcs_gadget_decode_params", "sjcs_add_compressed_particles",
+    "sjcs_add_compressed_particles_device",
     "sjcs_decode_compressed_device",
     "sjcs_server_create", "sjcs_server_submit", "sjcs_server_test", "sjcs_server_wait",
     "sjcs_server_times", "sjcs_server_release", "sjcs_server_error", "sjcs_server_destroy",
@@ -90,6 +91,38 @@ def read_compressed(path: str):
         if uv.size != 3 * n:
             raise ValueError(f"{path}: shorter than its header says")
     return hdr, uv.reshape(n, 3)
+
+
+def gadget_header(npart: int, npart_total: int, box: float = 1000.0, disp_min: float = -1.0,
+                  disp_max: float = 1.0, pos_bits: int = 16, id_start: int = 1) -> bytes:
+    """A 256-byte Gadget header (include/Gadget/GadgetHeader.h:4-27) with the
+    fields the decode reads: npart[1], npartTotal[1] (low word, an int) and
+    npartTotal[2] (high word), BoxSize, disp_min, disp_max, pos_bits,
+    id_start, id_end."""
+    import struct
+    h = bytearray(GADGET_HEADER_BYTES)
+    struct.pack_into("<6i", h, 0, 0, npart, 0, 0, 0, 0)
+    lo = npart_total & 0xFFFFFFFF
+    lo = lo - (1 << 32) if lo >= (1 << 31) else lo
+    struct.pack_into("<6i", h, 96, 0, lo, npart_total >> 32, 0, 0, 0)
+    struct.pack_into("<i", h, 124, 1)
+    struct.pack_into("<d", h, 128, box)
+    struct.pack_into("<ffi", h, 172, disp_min, disp_max, pos_bits)
+    struct.pack_into("<qq", h, 184, id_start, id_start + npart - 1)
+    return bytes(h)
+
+
+def write_compressed(path: str, header: bytes, uv) -> None:
+    """A compressed particle file: record markers around the header and
+    around the uint16 triplets (read back by read_compressed and by the
+    reference's readCompressedParticles)."""
+    import struct
+    uv = np.ascontiguousarray(uv, dtype="<u2").reshape(-1, 3)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", GADGET_HEADER_BYTES) + bytes(header) +
+                struct.pack("<i", GADGET_HEADER_BYTES) + struct.pack("<i", uv.nbytes & 0x7FFFFFFF))
+        uv.tofile(f)
+        f.write(struct.pack("<i", uv.nbytes & 0x7FFFFFFF))
 
 
 def gadget_decode_params(header: bytes, ntile: int = 1) -> GadgetDecode:
@@ -157,6 +190,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_copy_to_host": (i32, [vp, vp, vp, sz]),
         "sjcs_gadget_decode_params": (i32, [vp, i32, ctypes.POINTER(GadgetDecode)]),
         "sjcs_add_compressed_particles": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode)]),
+        "sjcs_add_compressed_particles_device": (i32, [vp, vp, u64,
+                                                       ctypes.POINTER(GadgetDecode)]),
         "sjcs_decode_compressed_device": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode), vp]),
     }
     for name, (res, args) in sig.items():
@@ -248,6 +283,10 @@ class Engine:
         uv = np.ascontiguousarray(uv, dtype=np.uint16).reshape(-1, 3)
         self._check(self._lib.sjcs_add_compressed_particles(self._h, _ptr(uv), uv.shape[0],
                                                             ctypes.byref(params)))
+
+    def add_compressed_particles_device(self, ptr: int, n: int, params: GadgetDecode):
+        self._check(self._lib.sjcs_add_compressed_particles_device(
+            self._h, ctypes.c_void_p(ptr), n, ctypes.byref(params)))
 
     def decode_compressed(self, uv, params: GadgetDecode) -> np.ndarray:
         """The device decode alone: triplets in, packed xyz out."""

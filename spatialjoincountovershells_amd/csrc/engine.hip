@@ -1021,6 +1021,27 @@ int sjcs_add_compressed_particles(sjcs_engine* e, const uint16_t* uv, uint64_t n
   return add_chunk(e, e->cin[sl].as<float>(), n);
 }
 
+int sjcs_add_compressed_particles_device(sjcs_engine* e, const uint16_t* d_uv, uint64_t n,
+                                         const sjcs_gadget_decode* p) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if ((!d_uv && n) || !p) return fail(e, SJCS_ERR_INVALID, "compressed particles: null pointer");
+  GadgetDecode d;
+  std::string err;
+  if (!gadget_prepare(*p, n, &d, &err)) return fail(e, SJCS_ERR_INVALID, err);
+  const int sl = e->slot;
+  // the decode rewrites the slot's staging once the sort that read it is done
+  if (e->cin[sl].bytes < 12 * (size_t)n)
+    HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+  HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
+  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
+  HIPCALL(e, decode_gadget(d_uv, n, d, e->cin[sl].as<float>(), e->bstream),
+          "decode compressed particles");
+  HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
+  return add_chunk(e, e->cin[sl].as<float>(), n);
+}
+
 int sjcs_decode_compressed_device(sjcs_engine* e, const uint16_t* d_uv, uint64_t n,
                                   const sjcs_gadget_decode* p, float* d_xyz) {
   if (!e) return SJCS_ERR_INVALID;

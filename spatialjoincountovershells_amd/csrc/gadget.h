@@ -32,6 +32,7 @@ struct GadgetDecode {
   float disp_min;
   double unitsep;
   int fused;
+  int one_tile;  // one tile per side and every id - 1 in [0, Ng3D)
 };
 
 // Fills d for the n particles of a file decoded with p.  Fails (false, err

@@ -16,37 +16,60 @@ namespace {
 constexpr int kDecodeThreads = 256;
 constexpr uint64_t kDecodeMaxBlocks = 1u << 16;
 
-// a / d truncated toward zero (C's integer division) for |a| < 2^50 and
-// 1 <= d, with inv = 1.0 / d rounded to double.  |a| * inv is within 1/4 of
-// |a| / d (two roundings, each 2^-53 relative, on a quotient below 2^50), so
-// its truncation is off by at most one, and the remainder's sign fixes that.
-__device__ __forceinline__ int64_t trunc_div(int64_t a, int64_t d, double inv) {
-  const bool neg = a < 0;
-  const int64_t m = neg ? -a : a;
-  int64_t q = (int64_t)((double)m * inv);
-  const int64_t r = m - q * d;
-  q += (r < 0) ? -1 : (r >= d ? 1 : 0);
-  return neg ? -q : q;
+// Integer division in double arithmetic.  Every id, quotient and remainder
+// here is an integer of magnitude below 2^50, so each is exact in a double:
+// q0 = trunc(a * inv), inv = 1/d rounded, is within 1/4 of a / d (two
+// roundings of 2^-53 each on a quotient below 2^50) and so off by at most
+// one; r = fma(-q0, d, a) is exact (q0*d < 2^51), and its sign fixes q0.
+// Cheaper on the vector ALU than 64-bit integer division or conversions.
+__device__ __forceinline__ void divmod_nonneg(double a, double d, double inv, double& q,
+                                              double& r) {
+  q = trunc(a * inv);
+  r = fma(-q, d, a);
+  if (r < 0.0) {
+    q -= 1.0;
+    r += d;
+  } else if (r >= d) {
+    q += 1.0;
+    r -= d;
+  }
 }
 
+// C's a / d and a % d: quotient truncated toward zero, remainder with the
+// sign of a.
+__device__ __forceinline__ void divmod_c(double a, double d, double inv, double& q, double& r) {
+  const bool neg = a < 0.0;
+  divmod_nonneg(fabs(a), d, inv, q, r);
+  if (neg) {
+    q = -q;
+    r = -r;
+  }
+}
+
+// kOneTile: one tile per side and 0 <= id - 1 < Ng3D for the whole file, so
+// the tile number is 0 and id - 1 is the number within the tile.
+template <bool kOneTile>
 __global__ __launch_bounds__(kDecodeThreads) void gadget_decode_kernel(
     const uint16_t* __restrict__ uv, uint64_t n, GadgetDecode g, float* __restrict__ xyz) {
   const uint64_t stride = (uint64_t)gridDim.x * kDecodeThreads;
+  const double ng1 = (double)g.ng1d, ng2 = (double)g.ng2d, ng3 = (double)g.ng3d;
+  const double base = (double)(g.id_start - 1);
   for (uint64_t i = (uint64_t)blockIdx.x * kDecodeThreads + threadIdx.x; i < n; i += stride) {
     // grid point of id = id_start + i (GadId.c:70-86, 112-137)
-    const int64_t m = g.id_start + (int64_t)i - 1;
-    const int64_t idt = trunc_div(m, g.ng3d, g.inv_ng3d);
-    const int64_t idg = m - idt * g.ng3d;
-    const int64_t ga = trunc_div(idg, g.ng2d, g.inv_ng2d);
-    const int64_t gr = idg - ga * g.ng2d;
-    const int64_t gb = trunc_div(gr, g.ng1d, g.inv_ng1d);
-    const int64_t gc = gr - gb * g.ng1d;
-    const int64_t ta = trunc_div(idt, g.nt2d, g.inv_nt2d);
-    const int64_t tr = idt - ta * g.nt2d;
-    const int64_t tb = trunc_div(tr, g.nt1d, g.inv_nt1d);
-    const int64_t tc = tr - tb * g.nt1d;
-    const int ip[3] = {(int)(ta * g.ng1d + ga), (int)(tb * g.ng1d + gb),
-                       (int)(tc * g.ng1d + gc)};
+    const double m = base + (double)i;
+    double ga, gb, gc, ta = 0.0, tb = 0.0, tc = 0.0, r;
+    if (kOneTile) {
+      divmod_nonneg(m, ng2, g.inv_ng2d, ga, r);
+      divmod_nonneg(r, ng1, g.inv_ng1d, gb, gc);
+    } else {
+      double idt, idg;
+      divmod_c(m, ng3, g.inv_ng3d, idt, idg);
+      divmod_c(idg, ng2, g.inv_ng2d, ga, r);
+      divmod_c(r, ng1, g.inv_ng1d, gb, gc);
+      divmod_c(idt, (double)g.nt2d, g.inv_nt2d, ta, r);
+      divmod_c(r, (double)g.nt1d, g.inv_nt1d, tb, tc);
+    }
+    const int ip[3] = {(int)fma(ta, ng1, ga), (int)fma(tb, ng1, gb), (int)fma(tc, ng1, gc)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float f = (float)ip[k];
@@ -91,6 +114,7 @@ bool gadget_prepare(const sjcs_gadget_decode& p, uint64_t n, GadgetDecode* d, st
   d->disp_min = p.disp_min;
   d->unitsep = p.unitsep;
   d->fused = p.fused ? 1 : 0;
+  d->one_tile = 0;
   if (n == 0) return true;
   // ids id_start .. id_start + n - 1, as m = id - 1
   if (n > (uint64_t)kIdMax || p.id_start < 2 - kIdMax || p.id_start > kIdMax) {
@@ -109,6 +133,7 @@ bool gadget_prepare(const sjcs_gadget_decode& p, uint64_t n, GadgetDecode* d, st
     *err = "compressed particles: ids give grid points beyond 2^31";
     return false;
   }
+  d->one_tile = (t1 == 1 && m_first >= 0 && m_last < d->ng3d) ? 1 : 0;
   return true;
 }
 
@@ -117,8 +142,12 @@ hipError_t decode_gadget(const uint16_t* uv, uint64_t n, const GadgetDecode& d, 
   if (n == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((n + kDecodeThreads - 1) / kDecodeThreads,
                                              kDecodeMaxBlocks);
-  hipLaunchKernelGGL(gadget_decode_kernel, dim3((unsigned)blocks), dim3(kDecodeThreads), 0, s, uv,
-                     n, d, xyz);
+  if (d.one_tile)
+    hipLaunchKernelGGL(gadget_decode_kernel<true>, dim3((unsigned)blocks), dim3(kDecodeThreads), 0,
+                       s, uv, n, d, xyz);
+  else
+    hipLaunchKernelGGL(gadget_decode_kernel<false>, dim3((unsigned)blocks), dim3(kDecodeThreads),
+                       0, s, uv, n, d, xyz);
   return hipGetLastError();
 }
 

@@ -8,6 +8,9 @@ uneven-work configuration (200M particles, 70% in 200k clumps with sigma
 uniform in [0.3, 2.0] and power-law sizes of exponent -2 truncated to
 [10, 1e5], rescaled so the clump members total 70% of the particles; 5M
 halos, 80% at clump centres).  Coordinates are wrapped into the periodic box.
+
+gadget_snapshot: a compressed (Gadget) snapshot of a side^3 grid whose
+particles carry uniform uint16 displacements over [-disp, disp].
 """
 from __future__ import annotations
 
@@ -66,3 +69,15 @@ def clustered(rng: np.random.Generator, n_particles: int, n_halos: int,
     halos[:nh_c] = centres[rng.integers(0, nclumps, nh_c)]
     halos[nh_c:] = uniform_points(rng, n_halos - nh_c)
     return parts, halos
+
+
+def gadget_snapshot(rng: np.random.Generator, side: int, disp: float = 20.0, chunk: int = 1 << 24):
+    """(header bytes, uint16 triplets [side^3][3]) of one file holding the
+    whole grid, ids 1..side^3."""
+    from . import gadget_header
+    n = side ** 3
+    uv = np.empty((n, 3), np.uint16)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        uv[a:b] = rng.integers(0, 65536, (b - a, 3), dtype=np.uint16)
+    return gadget_header(n, n, 1000.0, -disp, disp, 16, 1), uv
