@@ -167,6 +167,10 @@ int sjcs_sync(sjcs_engine *e);
 /* Copies the accumulated counts [nhalos][nrads] to `counts` (host).
  * Synchronises first. */
 int sjcs_get_counts(sjcs_engine *e, uint64_t *counts);
+/* The rows of halos [first, first + nhalos) only, into counts[nhalos][nrads]:
+ * a result writer can fetch the next rows while it formats these.  The rows
+ * are put back in input order once per state of the totals. */
+int sjcs_get_counts_range(sjcs_engine *e, uint64_t first, uint64_t nhalos, uint64_t *counts);
 
 /* Exact number of in-box (halo image, particle) pair tests the last chunk
  * made -- the reference's CalcDistance calls (MultiRangeCount.c:107).  Runs
@@ -261,6 +265,10 @@ int sjcs_device_alloc(sjcs_engine *e, size_t bytes, void **d_ptr);
 int sjcs_device_free(sjcs_engine *e, void *d_ptr);
 int sjcs_copy_to_device(sjcs_engine *e, void *d_dst, const void *src, size_t bytes);
 int sjcs_copy_to_host(sjcs_engine *e, void *dst, const void *d_src, size_t bytes);
+/* Page-locked host memory (fast transfers in both directions); no engine
+ * needed.  A failure's message is sjcs_last_error(NULL). */
+int sjcs_host_alloc(size_t bytes, void **out);
+void sjcs_host_free(void *p);
 
 /* ---- index: the ArraySTR lifecycle, one device-resident particle index ---- */
 

@@ -45,6 +45,7 @@ EXPORTED = (
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
+    "sjcs_host_alloc", "sjcs_host_free", "sjcs_get_counts_range",
     "sjcs_gadget_decode_params", "sjcs_add_compressed_particles",
     "sjcs_add_compressed_particles_device",
     "sjcs_decode_compressed_device",
@@ -188,6 +189,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_device_free": (i32, [vp, vp]),
         "sjcs_copy_to_device": (i32, [vp, vp, vp, sz]),
         "sjcs_copy_to_host": (i32, [vp, vp, vp, sz]),
+        "sjcs_host_alloc": (i32, [sz, pp]),
+        "sjcs_host_free": (None, [vp]),
+        "sjcs_get_counts_range": (i32, [vp, u64, u64, vp]),
         "sjcs_gadget_decode_params": (i32, [vp, i32, ctypes.POINTER(GadgetDecode)]),
         "sjcs_add_compressed_particles": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode)]),
         "sjcs_add_compressed_particles_device": (i32, [vp, vp, u64,
@@ -311,6 +315,12 @@ class Engine:
     def counts(self) -> np.ndarray:
         out = np.zeros((self.nhalos, self.nrads), dtype=np.uint64)
         self._check(self._lib.sjcs_get_counts(self._h, _ptr(out)))
+        return out
+
+    def counts_range(self, first: int, n: int) -> np.ndarray:
+        """Rows [first, first + n) of the counts."""
+        out = np.zeros((n, self.nrads), dtype=np.uint64)
+        self._check(self._lib.sjcs_get_counts_range(self._h, first, n, _ptr(out)))
         return out
 
     def last_chunk_pair_tests(self) -> int:

@@ -201,6 +201,8 @@ struct sjcs_engine {
   bool counts_fresh = true;  // the u32 counts hold nothing: next chunk stores
   bool c64_live = false;     // counts64 holds totals
   uint64_t pending = 0;      // bound on any u32 count
+  uint64_t counts_gen = 1;   // changes whenever the totals may change
+  uint64_t readout_gen = 0;  // totals `ucounts` holds (input row order)
 
 
   // Chunk i uses slot i % 2: its upload (bstream) overlaps the count of
@@ -390,7 +392,8 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
 }
 
 // Halo centres from a device AoS (packed xyz) -> Morton-sorted SoA + rows.
-int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n, bool tiles) {
+int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride, uint64_t n,
+               bool tiles) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 halos");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   DevBuf key[2], x[2], y[2], z[2], row[2];
@@ -405,7 +408,7 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n, bo
   }
   if (h == hipSuccess) rc = ensure_ws(e, radix_workspace_words(n));
   if (h == hipSuccess && rc == SJCS_OK)
-    h = halo_keys(d_xyz, n, key[0].as<uint32_t>(), x[0].as<float>(), y[0].as<float>(),
+    h = halo_keys(d_xyz, stride, n, key[0].as<uint32_t>(), x[0].as<float>(), y[0].as<float>(),
                   z[0].as<float>(), row[0].as<uint32_t>(), e->stream);
   int sel = 0;
   if (h == hipSuccess && rc == SJCS_OK) {
@@ -415,12 +418,21 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, uint64_t n, bo
         {x[1].as<uint32_t>(), y[1].as<uint32_t>(), z[1].as<uint32_t>(), row[1].as<uint32_t>()}};
     h = radix_sort_pairs(keys, pay, 4, n, 30, e->ws.as<uint32_t>(), e->stream, &sel);
   }
+  // the host looks for runs only when the device saw one
+  uint32_t runs = 0;
+  if (h == hipSuccess && rc == SJCS_OK && tiles && n >= kTileMin) {
+    h = ensure(e->inbox, 16);
+    if (h == hipSuccess)
+      h = has_key_runs(key[sel].as<uint32_t>(), n, kTileMin, e->inbox.as<uint32_t>(), e->stream);
+    if (h == hipSuccess)
+      h = hipMemcpyAsync(&runs, e->inbox.p, 4, hipMemcpyDeviceToHost, e->stream);
+  }
   if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
-  if (h != hipSuccess) rc = hip_fail(e, h, "halo sort");
+  if (h != hipSuccess && rc == SJCS_OK) rc = hip_fail(e, h, "halo sort");
   // runs of >= kTileMin halos with one Morton key go first, in order
   std::vector<uint32_t> rs, rl;
   uint64_t ntiled = 0;
-  if (rc == SJCS_OK && tiles && n >= kTileMin) {
+  if (rc == SJCS_OK && tiles && n >= kTileMin && runs) {
     std::vector<uint32_t> keys(n);
     h = hipMemcpy(keys.data(), key[sel].p, 4 * n, hipMemcpyDeviceToHost);
     if (h != hipSuccess) rc = hip_fail(e, h, "halo keys");
@@ -605,6 +617,7 @@ HaloView halo_view(const HaloState& h) {
 }
 
 int prepare_counts(sjcs_engine* e) {
+  ++e->counts_gen;
   e->counts_fresh = true;
   e->c64_live = false;
   e->pending = 0;
@@ -807,6 +820,7 @@ int piece_done(sjcs_engine* e, uint64_t n) {
   e->last = sl;
   e->slot = sl ^ 1;
   e->counts_fresh = false;
+  ++e->counts_gen;
   e->pending += images_max(e) * n;
   e->timed = true;
   return SJCS_OK;
@@ -960,8 +974,14 @@ int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uin
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   ++e->epoch;
-  rc = upload_points(e, e->in, points, stride_bytes, nhalos);
-  if (rc == SJCS_OK) rc = sort_halos(e, e->halos, e->in.as<float>(), nhalos, true);
+  // the records go up as they are (no host-side gather or page-locked
+  // staging); the key kernel reads the centres at their stride
+  // (the last record is read up to its centre only)
+  const size_t bytes = nhalos ? stride_bytes * (size_t)(nhalos - 1) + 12 : 0;
+  HIPCALL(e, ensure(e->in, bytes), "hipMalloc(halos)");
+  if (nhalos)
+    HIPCALL(e, hipMemcpy(e->in.p, points, bytes, hipMemcpyHostToDevice), "hipMemcpy(halos)");
+  rc = sort_halos(e, e->halos, e->in.as<float>(), stride_bytes, nhalos, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -974,7 +994,7 @@ int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   ++e->epoch;
-  rc = sort_halos(e, e->halos, d_xyz, nhalos, true);
+  rc = sort_halos(e, e->halos, d_xyz, 12, nhalos, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -1076,7 +1096,16 @@ int sjcs_sync(sjcs_engine* e) {
 int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
   int rc = check_ready(e);
   if (rc != SJCS_OK) return rc;
-  const size_t n = (size_t)e->halos.n * (size_t)e->shells.nrads;
+  return sjcs_get_counts_range(e, 0, e->halos.n, counts);
+}
+
+int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint64_t* counts) {
+  int rc = check_ready(e);
+  if (rc != SJCS_OK) return rc;
+  if (first > e->halos.n || nhalos > e->halos.n - first)
+    return fail(e, SJCS_ERR_INVALID, "counts: halo range out of bounds");
+  const size_t K = (size_t)e->shells.nrads;
+  const size_t n = (size_t)nhalos * K;
   if (!counts && n) return fail(e, SJCS_ERR_INVALID, "counts: null pointer");
   rc = quiesce(e);
   if (rc != SJCS_OK) return rc;
@@ -1085,14 +1114,20 @@ int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
     memset(counts, 0, n * sizeof(uint64_t));
     return SJCS_OK;
   }
-  HIPCALL(e, ensure(e->ucounts, n * sizeof(uint64_t)), "hipMalloc(counts)");
-  HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
-                            e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
-                            e->halos.row.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
-                            e->shells.nrads, e->stream),
-          "read out counts");
-  HIPCALL(e, hipMemcpyAsync(counts, e->ucounts.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                            e->stream),
+  // every row back in input order once per state of the totals; ranges are
+  // then plain copies
+  if (e->readout_gen != e->counts_gen) {
+    const size_t all = (size_t)e->halos.n * K;
+    HIPCALL(e, ensure(e->ucounts, all * sizeof(uint64_t)), "hipMalloc(counts)");
+    HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
+                              e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
+                              e->halos.row.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
+                              e->shells.nrads, e->stream),
+            "read out counts");
+    e->readout_gen = e->counts_gen;
+  }
+  HIPCALL(e, hipMemcpyAsync(counts, e->ucounts.as<uint64_t>() + first * K, n * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, e->stream),
           "hipMemcpy(counts)");
   HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
   return SJCS_OK;
@@ -1324,6 +1359,22 @@ int sjcs_copy_to_device(sjcs_engine* e, void* d_dst, const void* src, size_t byt
   return SJCS_OK;
 }
 
+int sjcs_host_alloc(size_t bytes, void** out) {
+  if (!out) return SJCS_ERR_INVALID;
+  *out = nullptr;
+  const hipError_t h = hipHostMalloc(out, bytes ? bytes : 16, hipHostMallocDefault);
+  if (h != hipSuccess) {
+    *out = nullptr;
+    g_create_error = std::string("hipHostMalloc: ") + hipGetErrorString(h);
+    return h == hipErrorOutOfMemory ? SJCS_ERR_NOMEM : SJCS_ERR_DEVICE;
+  }
+  return SJCS_OK;
+}
+
+void sjcs_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int sjcs_copy_to_host(sjcs_engine* e, void* dst, const void* d_src, size_t bytes) {
   if (!e || (!dst && bytes) || (!d_src && bytes)) return SJCS_ERR_INVALID;
   int rc = use_device(e);
@@ -1405,7 +1456,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   if (rc == SJCS_OK && !(sh.view.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
     rc = fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
   if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
-  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), n, false);
+  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), 12, n, false);
   const size_t nc = (size_t)n * (size_t)nrads;
   std::vector<uint64_t> host;
   if (rc == SJCS_OK) {
@@ -1586,7 +1637,7 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   SRVCALL(hipMemcpyAsync(sl.lut.p, r.lut.data(), sizeof(ShellEntry) * r.lut.size(),
                          hipMemcpyHostToDevice, sl.st),
           "hipMemcpy");
-  SRVCALL(halo_keys(sl.xyz.as<float>(), n, sl.key[0].as<uint32_t>(), sl.x[0].as<float>(),
+  SRVCALL(halo_keys(sl.xyz.as<float>(), 12, n, sl.key[0].as<uint32_t>(), sl.x[0].as<float>(),
                     sl.y[0].as<float>(), sl.z[0].as<float>(), sl.row[0].as<uint32_t>(), sl.st),
           "halo keys");
   int sel = 0;

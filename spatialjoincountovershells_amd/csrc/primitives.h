@@ -83,9 +83,15 @@ hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2]
 hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key,
                          float* x, float* y, float* z, hipStream_t s);
 
-// Halo centres (packed xyz) -> Morton keys (10 bits per axis) + SoA + row.
-hipError_t halo_keys(const float* xyz, uint64_t n, uint32_t* key, float* x,
+// Halo centres (3 FP32 every stride_bytes) -> Morton keys (10 bits per
+// axis) + SoA + row.
+hipError_t halo_keys(const float* pts, size_t stride_bytes, uint64_t n, uint32_t* key, float* x,
                      float* y, float* z, uint32_t* row, hipStream_t s);
+
+// *flag = 1 when the sorted keys hold a run of at least `run` equal keys,
+// else 0.
+hipError_t has_key_runs(const uint32_t* keys, uint64_t n, uint32_t run, uint32_t* flag,
+                        hipStream_t s);
 
 // Cell table from keys sorted ascending: cell_end[c] = #keys <= c.
 hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n,

@@ -648,14 +648,15 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   return v;
 }
 
-__global__ void halo_keys_kernel(const float* __restrict__ xyz, uint64_t n,
+__global__ void halo_keys_kernel(const char* __restrict__ pts, uint64_t stride, uint64_t n,
                                  uint32_t* __restrict__ key, float* __restrict__ x,
                                  float* __restrict__ y, float* __restrict__ z,
                                  uint32_t* __restrict__ row) {
   const float inv = 1024.0f / 1000.0f;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    float hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
+    const float* c = reinterpret_cast<const float*>(pts + i * stride);
+    float hx = c[0], hy = c[1], hz = c[2];
     uint32_t cx = (uint32_t)cell_of(hx, inv, 1023);
     uint32_t cy = (uint32_t)cell_of(hy, inv, 1023);
     uint32_t cz = (uint32_t)cell_of(hz, inv, 1023);
@@ -786,10 +787,28 @@ hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key, fl
   return hipGetLastError();
 }
 
-hipError_t halo_keys(const float* xyz, uint64_t n, uint32_t* key, float* x, float* y,
-                     float* z, uint32_t* row, hipStream_t s) {
+hipError_t halo_keys(const float* pts, size_t stride_bytes, uint64_t n, uint32_t* key, float* x,
+                     float* y, float* z, uint32_t* row, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  halo_keys_kernel<<<grid_blocks(n), 256, 0, s>>>(xyz, n, key, x, y, z, row);
+  halo_keys_kernel<<<grid_blocks(n), 256, 0, s>>>(reinterpret_cast<const char*>(pts),
+                                                  (uint64_t)stride_bytes, n, key, x, y, z, row);
+  return hipGetLastError();
+}
+
+// flag = 1 when keys[i] == keys[i + run - 1] for some i (a run of at least
+// `run` equal keys, the keys being sorted).
+__global__ void key_runs_kernel(const uint32_t* __restrict__ k, uint64_t n, uint32_t run,
+                                uint32_t* __restrict__ flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + run - 1 < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if (k[i] == k[i + run - 1]) *flag = 1u;
+}
+
+hipError_t has_key_runs(const uint32_t* keys, uint64_t n, uint32_t run, uint32_t* flag,
+                        hipStream_t s) {
+  hipError_t e = fill_u32(flag, 0u, 1, s);
+  if (e != hipSuccess || run < 1 || n < run) return e;
+  key_runs_kernel<<<grid_blocks(n), 256, 0, s>>>(keys, n, run, flag);
   return hipGetLastError();
 }
 

@@ -9,6 +9,18 @@
 
 #include "sjcs_host.h"
 
+int sjcs_read_compressed_header(const char *fname, sjcs_gadget_header *hdr)
+{
+    FILE *fp = fopen(fname, "r");
+    int rc = 0;
+    if (fp == NULL)
+        return -1;
+    if (fseek(fp, 4, SEEK_CUR) != 0 || fread(hdr, sizeof(*hdr), 1, fp) != 1 || hdr->npart[1] < 0)
+        rc = -2;
+    fclose(fp);
+    return rc;
+}
+
 int sjcs_read_compressed(const char *fname, sjcs_gadget_header *hdr, uint16_t **uv,
                          uint64_t *cap, uint64_t *n)
 {

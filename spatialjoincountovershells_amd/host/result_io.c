@@ -15,7 +15,9 @@
  *
  * The K label strings are formatted once; the counts are formatted by
  * worker threads into per-thread buffers, batch by batch, and written in
- * order.
+ * order.  The rows come from a callback, asked for the next batch while the
+ * workers format the current one (the device read-back overlaps the
+ * formatting); the totals are summed by the workers as they go.
  */
 #include <math.h>
 #include <pthread.h>
@@ -25,10 +27,10 @@
 
 #include "sjcs_host.h"
 
-#define BATCH_HALOS 32768
+#define BATCH_HALOS SJCS_RESULT_BATCH
 
 typedef struct {
-    char *s;
+    char *s;     /* at least LABEL_SLOT bytes allocated */
     size_t len;
 } label_t;
 
@@ -38,11 +40,28 @@ static const char digits2[] =
     "8081828384858687888990919293949596979899";
 
 /* Decimal of v at p, as %lu prints it; returns the end. */
-static char *put_u64(char *p, uint64_t v)
+static inline char *put_u64(char *p, uint64_t v)
 {
-    char tmp[24];
-    char *e = tmp + sizeof(tmp);
-    char *q = e;
+    int nd;
+    char *q;
+    if (v < 10) {
+        *p = (char)('0' + v);
+        return p + 1;
+    }
+    if (v < 100) {
+        p[0] = digits2[2 * v];
+        p[1] = digits2[2 * v + 1];
+        return p + 2;
+    }
+    nd = 3;
+    {
+        uint64_t t = v / 1000;
+        while (t) {
+            nd++;
+            t /= 10;
+        }
+    }
+    q = p + nd;
     while (v >= 100) {
         unsigned r = (unsigned)(v % 100);
         v /= 100;
@@ -57,23 +76,68 @@ static char *put_u64(char *p, uint64_t v)
     } else {
         *--q = (char)('0' + v);
     }
-    memcpy(p, q, (size_t)(e - q));
-    return p + (e - q);
+    return p + nd;
 }
+
+/* A label of at most LABEL_SLOT bytes is copied as one fixed-size block
+ * (the bytes past its end are overwritten by what follows). */
+#define LABEL_SLOT 32
 
 typedef struct {
     const sjcs_halos *h;
-    const uint64_t *counts;
+    const uint64_t *rows;   /* rows of the batch, from halo `first` */
+    uint64_t first;
     const label_t *labels;
     int nrads;
     uint64_t beg, end;
+    uint64_t *tot;          /* this worker's share of the totals */
     char *buf;
     size_t cap, len;
 } fmt_job;
 
-static void *fmt_run(void *arg)
+static void fmt_batch(fmt_job *j);
+
+/* Workers that stay up for the whole result: the main thread bumps `gen`
+ * for each batch, every worker formats its job and the last one to finish
+ * wakes the main thread. */
+typedef struct {
+    pthread_mutex_t mu;
+    pthread_cond_t go, finished;
+    uint64_t gen;
+    int remaining, stop;
+} pool_t;
+
+typedef struct {
+    pool_t *pool;
+    fmt_job *job;
+} worker_arg;
+
+static void *pool_run(void *arg)
 {
-    fmt_job *j = arg;
+    worker_arg *w = arg;
+    pool_t *pl = w->pool;
+    uint64_t seen = 0;
+    for (;;) {
+        pthread_mutex_lock(&pl->mu);
+        while (pl->gen == seen && !pl->stop)
+            pthread_cond_wait(&pl->go, &pl->mu);
+        if (pl->stop) {
+            pthread_mutex_unlock(&pl->mu);
+            break;
+        }
+        seen = pl->gen;
+        pthread_mutex_unlock(&pl->mu);
+        fmt_batch(w->job);
+        pthread_mutex_lock(&pl->mu);
+        if (--pl->remaining == 0)
+            pthread_cond_signal(&pl->finished);
+        pthread_mutex_unlock(&pl->mu);
+    }
+    return NULL;
+}
+
+static void fmt_batch(fmt_job *j)
+{
     size_t maxline = 0, need;
     uint64_t n;
     char *p;
@@ -81,7 +145,7 @@ static void *fmt_run(void *arg)
     for (k = 0; k < j->nrads; k++)
         if (j->labels[k].len > maxline)
             maxline = j->labels[k].len;
-    need = (j->end - j->beg) * ((size_t)j->nrads * (maxline + 22) + 24) + 1;
+    need = (j->end - j->beg) * ((size_t)j->nrads * (maxline + 22) + 24) + LABEL_SLOT + 1;
     if (need > j->cap) {
         free(j->buf);
         j->buf = malloc(need);
@@ -89,35 +153,61 @@ static void *fmt_run(void *arg)
     }
     if (j->buf == NULL) {
         j->len = (size_t)-1;
-        return NULL;
+        return;
     }
     p = j->buf;
     for (n = j->beg; n < j->end; n++) {
-        const uint64_t *row = j->counts + n * (uint64_t)j->nrads;
+        const uint64_t *row = j->rows + (n - j->first) * (uint64_t)j->nrads;
         memcpy(p, "id ", 3);
         p = put_u64(p + 3, j->h->halo[n].id);
         *p++ = '\n';
         for (k = 0; k < j->nrads; k++) {
-            memcpy(p, j->labels[k].s, j->labels[k].len);
-            p = put_u64(p + j->labels[k].len, row[k]);
+            const label_t *l = &j->labels[k];
+            if (l->len <= LABEL_SLOT)
+                __builtin_memcpy(p, l->s, LABEL_SLOT);
+            else
+                memcpy(p, l->s, l->len);
+            p = put_u64(p + l->len, row[k]);
             *p++ = '\n';
+            j->tot[k] += row[k];
         }
     }
     j->len = (size_t)(p - j->buf);
-    return NULL;
+}
+
+typedef struct {
+    const uint64_t *counts;
+    int nrads;
+} array_rows;
+
+static const uint64_t *rows_of_array(void *ctx, uint64_t beg, uint64_t n)
+{
+    const array_rows *a = ctx;
+    (void)n;
+    return a->counts + beg * (uint64_t)a->nrads;
 }
 
 int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *counts,
                       const float *radiuses_sq, int nrads, int nthreads)
 {
+    array_rows a = {counts, nrads};
+    return sjcs_write_result_rows(fname, h, rows_of_array, &a, radiuses_sq, nrads, nthreads);
+}
+
+int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows, void *ctx,
+                           const float *radiuses_sq, int nrads, int nthreads)
+{
     label_t *labels;
-    uint64_t *total;
+    uint64_t *total, *row0;
+    const uint64_t *cur, *next;
     fmt_job *jobs;
     pthread_t *th;
+    worker_arg *wargs;
+    pool_t pool;
     FILE *fp;
     char msg[1024];
-    uint64_t b, n;
-    int k, t, rc = 0;
+    uint64_t b;
+    int k, t, nworkers, rc = 0;
 
     snprintf(msg, sizeof(msg), "ERROR: while writing %s in %s()\n", fname ? fname : "(null)",
              "outputResult");
@@ -139,7 +229,7 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
             else
                 need = (size_t)snprintf(NULL, 0, "  [%f ~ %f]: ",
                                         (float)sqrt((double)radiuses_sq[k - 1]), r) + 1;
-            labels[k].s = malloc(need);
+            labels[k].s = calloc(need > LABEL_SLOT ? need : LABEL_SLOT, 1);
             if (k == 0)
                 snprintf(labels[k].s, need, "  [0.0 ~ %f]: ", r);
             else
@@ -147,7 +237,7 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
                          (float)sqrt((double)radiuses_sq[k - 1]), r);
             labels[k].len = need - 1;
         } else {
-            labels[k].s = malloc((size_t)len + 1);
+            labels[k].s = calloc((size_t)len + 1 > LABEL_SLOT ? (size_t)len + 1 : LABEL_SLOT, 1);
             memcpy(labels[k].s, tmp, (size_t)len + 1);
             labels[k].len = (size_t)len;
         }
@@ -166,32 +256,69 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
         }
     }
 
-    /* totals: every row once, halo 0's row twice (main.c:400,413) */
+    /* totals: every row once, halo 0's row twice (main.c:400,413); the
+     * workers sum the rows they format */
     total = calloc((size_t)nrads, sizeof(uint64_t));
-    for (n = 0; n < h->n; n++)
-        for (k = 0; k < nrads; k++)
-            total[k] += counts[n * (uint64_t)nrads + k];
-    if (h->n > 0)
-        for (k = 0; k < nrads; k++)
-            total[k] += counts[k];
+    row0 = calloc((size_t)nrads, sizeof(uint64_t));
 
     if (nthreads < 1)
         nthreads = 1;
     jobs = calloc((size_t)nthreads, sizeof(*jobs));
     th = calloc((size_t)nthreads, sizeof(*th));
+    wargs = calloc((size_t)nthreads, sizeof(*wargs));
+    for (t = 0; t < nthreads; t++)
+        jobs[t].tot = calloc((size_t)nrads, sizeof(uint64_t));
+    pthread_mutex_init(&pool.mu, NULL);
+    pthread_cond_init(&pool.go, NULL);
+    pthread_cond_init(&pool.finished, NULL);
+    pool.gen = 0;
+    pool.remaining = 0;
+    pool.stop = 0;
+    /* jobs [0, nworkers) run on workers, the rest (if a thread could not be
+     * started) on this thread */
+    nworkers = 0;
+    for (t = 0; t < nthreads; t++) {
+        wargs[t].pool = &pool;
+        wargs[t].job = &jobs[t];
+        if (pthread_create(&th[t], NULL, pool_run, &wargs[t]) != 0)
+            break;
+        nworkers++;
+    }
+    cur = h->n ? rows(ctx, 0, h->n < BATCH_HALOS ? h->n : BATCH_HALOS) : NULL;
+    if (h->n && cur == NULL)
+        rc = -1;
+    else if (h->n)
+        memcpy(row0, cur, sizeof(uint64_t) * (size_t)nrads);
     for (b = 0; b < h->n && rc == 0; b += BATCH_HALOS) {
         uint64_t e = b + BATCH_HALOS < h->n ? b + BATCH_HALOS : h->n;
         for (t = 0; t < nthreads; t++) {
             jobs[t].h = h;
-            jobs[t].counts = counts;
+            jobs[t].rows = cur;
+            jobs[t].first = b;
             jobs[t].labels = labels;
             jobs[t].nrads = nrads;
             jobs[t].beg = b + (e - b) * (uint64_t)t / (uint64_t)nthreads;
             jobs[t].end = b + (e - b) * (uint64_t)(t + 1) / (uint64_t)nthreads;
-            pthread_create(&th[t], NULL, fmt_run, &jobs[t]);
         }
-        for (t = 0; t < nthreads; t++)
-            pthread_join(th[t], NULL);
+        pthread_mutex_lock(&pool.mu);
+        pool.remaining = nworkers;
+        pool.gen++;
+        pthread_cond_broadcast(&pool.go);
+        pthread_mutex_unlock(&pool.mu);
+        /* the next batch's rows arrive while this one is formatted */
+        next = NULL;
+        if (e < h->n) {
+            uint64_t e2 = e + BATCH_HALOS < h->n ? e + BATCH_HALOS : h->n;
+            next = rows(ctx, e, e2 - e);
+            if (next == NULL)
+                rc = -1;
+        }
+        for (t = nworkers; t < nthreads; t++)
+            fmt_batch(&jobs[t]);
+        pthread_mutex_lock(&pool.mu);
+        while (pool.remaining > 0)
+            pthread_cond_wait(&pool.finished, &pool.mu);
+        pthread_mutex_unlock(&pool.mu);
         for (t = 0; t < nthreads && rc == 0; t++) {
             if (jobs[t].len == (size_t)-1 ||
                 fwrite(jobs[t].buf, 1, jobs[t].len, fp) != jobs[t].len || ferror(fp)) {
@@ -199,7 +326,23 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
                 rc = -1;
             }
         }
+        cur = next;
     }
+    pthread_mutex_lock(&pool.mu);
+    pool.stop = 1;
+    pthread_cond_broadcast(&pool.go);
+    pthread_mutex_unlock(&pool.mu);
+    for (t = 0; t < nworkers; t++)
+        pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&pool.mu);
+    pthread_cond_destroy(&pool.go);
+    pthread_cond_destroy(&pool.finished);
+    for (t = 0; t < nthreads; t++)
+        for (k = 0; k < nrads; k++)
+            total[k] += jobs[t].tot[k];
+    if (h->n > 0)
+        for (k = 0; k < nrads; k++)
+            total[k] += row0[k];
     if (rc == 0) {
         fputs("\nTotal\n", fp);
         for (k = 0; k < nrads; k++) {
@@ -223,11 +366,15 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
     } else {
         fflush(fp);
     }
-    for (t = 0; t < nthreads; t++)
+    for (t = 0; t < nthreads; t++) {
         free(jobs[t].buf);
+        free(jobs[t].tot);
+    }
     free(jobs);
     free(th);
+    free(wargs);
     free(total);
+    free(row0);
     for (k = 0; k < nrads; k++)
         free(labels[k].s);
     free(labels);

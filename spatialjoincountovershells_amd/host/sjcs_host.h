@@ -102,6 +102,9 @@ void sjcs_free_halos(sjcs_halos *h);
 int sjcs_particle_count(const char *fname, uint64_t *n);
 /* Reads the first n particles (3 x FP32 each) into dst. */
 int sjcs_read_particles(const char *fname, float *dst, uint64_t n);
+/* bytes [offset, offset + bytes) of a file into dst with nthreads parallel
+ * preads.  0, -1 (cannot open), -2 (shorter). */
+int sjcs_pread_par(const char *fname, uint64_t offset, uint64_t bytes, void *dst, int nthreads);
 
 /* ---- compressed particles (src/Gadget/BufVector3D.c:111-136) ---- */
 
@@ -113,6 +116,10 @@ int sjcs_read_particles(const char *fname, float *dst, uint64_t n);
  * memory runs out. */
 int sjcs_read_compressed(const char *fname, sjcs_gadget_header *hdr, uint16_t **uv,
                          uint64_t *cap, uint64_t *n);
+/* The header alone; the triplets start at byte SJCS_GADGET_DATA_OFFSET.
+ * 0, -1 (cannot open), -2 (shorter than a header, or npart[1] < 0). */
+int sjcs_read_compressed_header(const char *fname, sjcs_gadget_header *hdr);
+#define SJCS_GADGET_DATA_OFFSET 268
 
 /* ---- result (src/astr_rcount/main.c:371-442) ---- */
 
@@ -121,6 +128,15 @@ int sjcs_read_compressed(const char *fname, sjcs_gadget_header *hdr, uint16_t **
  * fname NULL = stdout.  Returns 0 or -1. */
 int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *counts,
                       const float *radiuses_sq, int nrads, int nthreads);
+/* The same with the count rows from a callback: rows(ctx, beg, n) returns
+ * the rows of halos [beg, beg + n) (n * nrads u64), or NULL on failure; the
+ * writer asks for the next batch while it formats the current one, so a
+ * returned block must stay valid until the call after next. */
+typedef const uint64_t *(*sjcs_rows_fn)(void *ctx, uint64_t beg, uint64_t n);
+int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows, void *ctx,
+                           const float *radiuses_sq, int nrads, int nthreads);
+/* Halos per batch of the writer (the largest n it asks rows for). */
+#define SJCS_RESULT_BATCH 32768
 
 /* milliseconds between two gettimeofday samples, truncated as the
  * reference's printTimeMillis does (include/time.h:30-39) */

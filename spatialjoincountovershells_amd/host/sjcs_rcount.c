@@ -6,17 +6,27 @@
  * bad options, same text result.  What runs differs: the halos, shell
  * thresholds and u64 counts live on the GPU (include/sjcs.h); each particle
  * file is indexed (cell sort) and counted there, and the next file is read
- * from disk while the GPU works on the current one.  The per-file
- * "index construction" line reports upload + cell sort, "searching" the
- * count kernel, both from HIP events; the index buffers are reused from
- * file to file, so "index destruction" is ~0.  With --compressed, each
- * file's uint16 triplets are uploaded as they are and decoded on the
- * device ("particles decoding + index construction" = upload + decode +
- * cell sort).
+ * from disk while the GPU works on the current one.
+ *
+ * Host side, in order of the wall time it saves:
+ *  - the device is set up (runtime start, engine, pinned staging) on a
+ *    thread of its own while the halo file is parsed;
+ *  - a particle file goes from disk to the device in 64 MiB pieces, each
+ *    read by parallel preads into page-locked memory and copied from there;
+ *  - the counts come back in batches of rows that the result writer asks
+ *    for while its workers format the previous batch.
+ * The per-file "particles reading time" line reports disk to device memory,
+ * "index construction" the cell sort, "searching" the count kernel, the
+ * last two from HIP events; the index buffers are reused from file to file,
+ * so "index destruction" is ~0.  With --compressed, each file's uint16
+ * triplets go to the device as they are and are decoded there ("particles
+ * decoding + index construction" = decode + cell sort).
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <sys/time.h>
 #include <unistd.h>
 
@@ -24,6 +34,23 @@
 #include "sjcs_host.h"
 
 #define TIMER_START(tv) gettimeofday(&(tv), NULL)
+#define PIECE_BYTES ((uint64_t)64 << 20)
+
+/* SJCS_TRACE=1: milliseconds since start at each step, on stderr (the
+ * reference's timer lines leave process start-up, device set-up, count
+ * read-back and clean-up out). */
+static struct timeval g_t0;
+static int g_trace;
+
+static void trace(const char *what)
+{
+    struct timeval b;
+    if (!g_trace)
+        return;
+    gettimeofday(&b, NULL);
+    fprintf(stderr, "[sjcs trace] %8.1f ms  %s\n",
+            (b.tv_sec - g_t0.tv_sec) * 1e3 + (b.tv_usec - g_t0.tv_usec) * 1e-3, what);
+}
 
 static void print_ms(const char *what, struct timeval *a)
 {
@@ -38,69 +65,156 @@ static void die_engine(sjcs_engine *e, const char *what)
     exit(EXIT_FAILURE);
 }
 
-typedef struct {
-    float *xyz;
-    uint16_t *uv;             /* --compressed: the file's triplets */
-    sjcs_gadget_decode dec;   /* --compressed: their decode parameters */
-    uint64_t n, cap;
-    long read_ms;
-} chunk_buf;
+/* ---- device set-up, overlapped with the halo parse ---- */
 
-/* Reads compressed particle file fname into c (main.c:739-756). */
-static void read_compressed_chunk(const char *fname, chunk_buf *c)
-{
-    struct timeval a, b;
-    sjcs_gadget_header hdr;
-    uint64_t n = 0;
+typedef struct chunk_buf chunk_buf;
+static int read_chunk(sjcs_engine *e, void *pin[2], const char *fname, chunk_buf *c,
+                      int compressed, int nthreads, char *err, size_t errlen);
+
+typedef struct {
+    int device;
+    sjcs_engine *e;
+    void *pin[2];  /* page-locked pieces of a particle file */
     int rc;
-    TIMER_START(a);
-    rc = sjcs_read_compressed(fname, &hdr, &c->uv, &c->cap, &n);
-    if (rc == -1) {
-        fprintf(stderr, "ERROR: file %s not found in %s()\n", fname, "readCompressedParticles");
-        exit(EXIT_FAILURE);
-    }
-    if (rc != 0) {
-        fprintf(stderr, "ERROR: %s in %s()\n",
-                rc == -3 ? "memory shortage" : "file shorter than its header says",
-                "readCompressedParticles");
-        exit(EXIT_FAILURE);
-    }
-    /* one tile per side, as constructIndex* fixes it (main.c:744) */
-    if (sjcs_gadget_decode_params(&hdr, 1, &c->dec) != SJCS_OK) {
-        fprintf(stderr, "ERROR: bad particle totals in the header of %s\n", fname);
-        exit(EXIT_FAILURE);
-    }
-    c->n = n;
-    gettimeofday(&b, NULL);
-    c->read_ms = sjcs_elapsed_ms(&a, &b);
+    /* the first particle file, read while the halos are parsed */
+    const char *first;
+    chunk_buf *buf;
+    int compressed, nthreads;
+    int read_rc;
+    char err[4096 + 256];
+} setup_job;
+
+static void *setup_run(void *arg)
+{
+    setup_job *s = arg;
+    s->rc = sjcs_engine_create(s->device, &s->e);
+    if (s->rc == SJCS_OK)
+        s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[0]);
+    if (s->rc == SJCS_OK)
+        s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[1]);
+    if (s->rc == SJCS_OK)
+        s->read_rc = read_chunk(s->e, s->pin, s->first, s->buf, s->compressed, s->nthreads,
+                                s->err, sizeof(s->err));
+    return NULL;
 }
 
-/* Reads particle file fname into c; exits like the reference would fail. */
-static void read_chunk(const char *fname, chunk_buf *c)
+/* ---- particle files: disk -> device ---- */
+
+struct chunk_buf {
+    void *d;               /* device buffer */
+    uint64_t cap;          /* its bytes */
+    uint64_t n;            /* particles of the file in it */
+    sjcs_gadget_decode dec;
+    long read_ms;
+};
+
+/* Bytes [off, off + bytes) of fname into device memory at d, through the
+ * two pinned pieces.  0, the sjcs_pread_par status, or -3 (copy failed). */
+static int file_to_device(sjcs_engine *e, void *pin[2], const char *fname, uint64_t off,
+                          uint64_t bytes, char *d, int nthreads)
+{
+    uint64_t done = 0;
+    int k = 0;
+    while (done < bytes) {
+        uint64_t len = bytes - done < PIECE_BYTES ? bytes - done : PIECE_BYTES;
+        int rc = sjcs_pread_par(fname, off + done, len, pin[k], nthreads);
+        if (rc != 0)
+            return rc;
+        if (sjcs_copy_to_device(e, d + done, pin[k], (size_t)len) != SJCS_OK)
+            return -3;
+        done += len;
+        k ^= 1;
+    }
+    return 0;
+}
+
+static int ensure_device(sjcs_engine *e, chunk_buf *c, uint64_t bytes)
+{
+    if (bytes == 0)
+        bytes = 16;
+    if (c->cap >= bytes && c->d != NULL)
+        return 0;
+    if (c->d)
+        sjcs_device_free(e, c->d);
+    c->d = NULL;
+    c->cap = 0;
+    if (sjcs_device_alloc(e, (size_t)bytes, &c->d) != SJCS_OK)
+        return -1;
+    c->cap = bytes;
+    return 0;
+}
+
+/* Reads particle file fname into c's device buffer.  0, or -1 with the
+ * message the reference's failure would print in err (src/Gadget/
+ * BufVector3D.c:111-136, 285-308). */
+static int read_chunk(sjcs_engine *e, void *pin[2], const char *fname, chunk_buf *c,
+                      int compressed, int nthreads, char *err, size_t errlen)
 {
     struct timeval a, b;
-    uint64_t n = 0;
+    uint64_t n = 0, off, bytes;
+    const char *fn = compressed ? "readCompressedParticles" : "readDecodedParticles";
+    int rc;
     TIMER_START(a);
-    if (sjcs_particle_count(fname, &n) != 0) {
-        fprintf(stderr, "ERROR: file %s not found in %s()\n", fname, "readDecodedParticles");
-        exit(EXIT_FAILURE);
-    }
-    if (n > c->cap) {
-        free(c->xyz);
-        c->xyz = malloc(12 * (size_t)n);
-        if (c->xyz == NULL) {
-            fprintf(stderr, "ERROR: memory shortage in %s()\n", "readDecodedParticles");
-            exit(EXIT_FAILURE);
+    if (compressed) {
+        sjcs_gadget_header hdr;
+        struct stat st;
+        rc = sjcs_read_compressed_header(fname, &hdr);
+        if (rc == -1) {
+            snprintf(err, errlen, "ERROR: file %s not found in %s()\n", fname, fn);
+            return -1;
         }
-        c->cap = n;
+        n = rc == 0 ? (uint64_t)hdr.npart[1] : 0;
+        if (rc != 0 || stat(fname, &st) != 0 ||
+            (uint64_t)st.st_size < SJCS_GADGET_DATA_OFFSET + 6 * n) {
+            snprintf(err, errlen, "ERROR: file shorter than its header says in %s()\n", fn);
+            return -1;
+        }
+        /* one tile per side, as constructIndex* fixes it (main.c:744) */
+        if (sjcs_gadget_decode_params(&hdr, 1, &c->dec) != SJCS_OK) {
+            snprintf(err, errlen, "ERROR: bad particle totals in the header of %s\n", fname);
+            return -1;
+        }
+        off = SJCS_GADGET_DATA_OFFSET;
+        bytes = 6 * n;
+    } else {
+        if (sjcs_particle_count(fname, &n) != 0) {
+            snprintf(err, errlen, "ERROR: file %s not found in %s()\n", fname, fn);
+            return -1;
+        }
+        off = 0;
+        bytes = 12 * n;
     }
-    if (n && sjcs_read_particles(fname, c->xyz, n) != 0) {
-        fprintf(stderr, "ERROR: while reading %s in %s()\n", fname, "readDecodedParticles");
-        exit(EXIT_FAILURE);
+    if (ensure_device(e, c, bytes) != 0) {
+        snprintf(err, errlen, "ERROR: device memory for %s: %s\n", fname, sjcs_last_error(e));
+        return -1;
+    }
+    rc = file_to_device(e, pin, fname, off, bytes, c->d, nthreads);
+    if (rc == -1) {
+        snprintf(err, errlen, "ERROR: file %s not found in %s()\n", fname, fn);
+        return -1;
+    }
+    if (rc == -3) {
+        snprintf(err, errlen, "ERROR: particle upload: %s\n", sjcs_last_error(e));
+        return -1;
+    }
+    if (rc != 0) {
+        snprintf(err, errlen, "ERROR: while reading %s in %s()\n", fname, fn);
+        return -1;
     }
     c->n = n;
     gettimeofday(&b, NULL);
     c->read_ms = sjcs_elapsed_ms(&a, &b);
+    return 0;
+}
+
+static void read_chunk_or_die(sjcs_engine *e, void *pin[2], const char *fname, chunk_buf *c,
+                              int compressed, int nthreads)
+{
+    char err[4096 + 256];
+    if (read_chunk(e, pin, fname, c, compressed, nthreads, err, sizeof(err)) != 0) {
+        fputs(err, stderr);
+        exit(EXIT_FAILURE);
+    }
 }
 
 /* Waits for the chunk in flight and prints its lines. */
@@ -119,12 +233,23 @@ static void finish_chunk(sjcs_engine *e, uint64_t n, int compressed)
     print_ms(">>>> index destruction time", &a);
 }
 
-static void read_any(const char *fname, chunk_buf *c, int compressed)
+/* ---- counts -> result writer ---- */
+
+typedef struct {
+    sjcs_engine *e;
+    uint64_t *rows[2];  /* page-locked, one batch of rows each */
+    int slot;
+} row_source;
+
+static const uint64_t *fetch_rows(void *ctx, uint64_t beg, uint64_t n)
 {
-    if (compressed)
-        read_compressed_chunk(fname, c);
-    else
-        read_chunk(fname, c);
+    row_source *r = ctx;
+    r->slot ^= 1;
+    if (sjcs_get_counts_range(r->e, beg, n, r->rows[r->slot]) != SJCS_OK) {
+        fprintf(stderr, "ERROR: counts: %s\n", sjcs_last_error(r->e));
+        return NULL;
+    }
+    return r->rows[r->slot];
 }
 
 int main(int argc, char *argv[])
@@ -135,11 +260,16 @@ int main(int argc, char *argv[])
     struct timeval tv, tvtotal;
     char err[4096];
     float *rad, *r2;
-    uint64_t *counts;
     chunk_buf bufs[2];
-    int nfiles, i, hthreads, device = 0, rc;
+    setup_job setup;
+    pthread_t setup_th;
+    row_source src;
+    int nfiles, i, hthreads, rc, setup_threaded;
     const char *dev_env = getenv("SJCS_DEVICE");
+    const char *trace_env = getenv("SJCS_TRACE");
 
+    gettimeofday(&g_t0, NULL);
+    g_trace = trace_env != NULL && trace_env[0] == '1';
     sjcs_cli_parse(argc, argv, &cli);
     sjcs_cli_print_info(stdout, &cli, argv);
     if (cli.errflag) {
@@ -148,9 +278,17 @@ int main(int argc, char *argv[])
         exit(EXIT_FAILURE);
     }
     memset(bufs, 0, sizeof(bufs));
+    memset(&setup, 0, sizeof(setup));
     hthreads = cli.nthreads < 256 ? cli.nthreads : 256;
-    if (dev_env)
-        device = atoi(dev_env);
+    setup.device = dev_env ? atoi(dev_env) : 0;
+    setup.first = argv[cli.particle_beg];
+    setup.buf = &bufs[0];
+    setup.compressed = cli.compressed;
+    /* a share of the threads: the halo parse runs meanwhile */
+    setup.nthreads = hthreads / 4 > 0 ? hthreads / 4 : 1;
+    setup_threaded = pthread_create(&setup_th, NULL, setup_run, &setup) == 0;
+    if (!setup_threaded)
+        setup_run(&setup);
 
     /* halos (main.c:634-642) */
     TIMER_START(tv);
@@ -164,65 +302,90 @@ int main(int argc, char *argv[])
     }
     print_ms(">>>> halo data reading time", &tv);
     printf("  # of halos = %lu\n", (unsigned long)halos.n);
+    trace("halos read");
 
-    /* shells (main.c:645-651) */
+    /* shells (main.c:645-649) */
     rad = malloc(sizeof(float) * (size_t)cli.nrads);
     r2 = malloc(sizeof(float) * (size_t)cli.nrads);
-    counts = calloc((size_t)halos.n * (size_t)cli.nrads + 1, sizeof(uint64_t));
-    if (!rad || !r2 || !counts) {
+    if (!rad || !r2) {
         fprintf(stderr, "ERROR: memory shortage\n");
         exit(EXIT_FAILURE);
     }
     sjcs_host_shell_thresholds(cli.minrad, cli.maxrad, cli.nrads, rad, r2);
 
-    if (sjcs_engine_create(device, &e) != SJCS_OK)
-        die_engine(NULL, "GPU engine");
+    if (setup_threaded)
+        pthread_join(setup_th, NULL);
+    e = setup.e;
+    if (setup.rc != SJCS_OK)
+        die_engine(e, "GPU engine");
+    trace("device set up");
     if (sjcs_set_shells(e, r2, cli.nrads) != SJCS_OK)
         die_engine(e, "shells");
     if (sjcs_set_halos(e, halos.n ? halos.halo[0].point : NULL, sizeof(sjcs_halo), halos.n) !=
         SJCS_OK)
         die_engine(e, "halos");
+    trace("shells and halos on the device");
 
     /* chunk loop (main.c:235-368): file i+1 is read while file i counts */
     nfiles = cli.particle_end - cli.particle_beg;
     TIMER_START(tvtotal);
     puts(cli.maxbacklogs == 0 ? "  -- with no backlog execution" : "  -- with backlog execution");
     fflush(stdout);
-    read_any(argv[cli.particle_beg], &bufs[0], cli.compressed);
+    if (setup.read_rc != 0) {
+        fputs(setup.err, stderr);
+        exit(EXIT_FAILURE);
+    }
     for (i = 0; i < nfiles; i++) {
         chunk_buf *cur = &bufs[i & 1];
         printf(">>>> particles reading time: %ld milli seconds\n", cur->read_ms);
         if (cur->n == 0)
             fprintf(stderr, "%s(): No data loaded. Cannot make tree.\n",
                     "MakeTreeWithSameThreadPoolArraySTR");
-        rc = cli.compressed ? sjcs_add_compressed_particles(e, cur->uv, cur->n, &cur->dec)
-                            : sjcs_add_particles(e, cur->xyz, cur->n);
+        rc = cli.compressed
+                 ? sjcs_add_compressed_particles_device(e, cur->d, cur->n, &cur->dec)
+                 : sjcs_add_particles_device(e, cur->d, cur->n);
         if (rc != SJCS_OK)
             die_engine(e, "particle chunk");
-        if (i + 1 < nfiles)
-            read_any(argv[cli.particle_beg + i + 1], &bufs[(i + 1) & 1], cli.compressed);
+        trace("chunk queued");
+        if (i + 1 < nfiles) {
+            read_chunk_or_die(e, setup.pin, argv[cli.particle_beg + i + 1], &bufs[(i + 1) & 1],
+                              cli.compressed, hthreads);
+            trace("next file on the device");
+        }
         finish_chunk(e, cur->n, cli.compressed);
+        trace("chunk counted");
         fflush(stdout);
     }
     print_ms(">>>> total searching time", &tvtotal);
 
-    if (sjcs_get_counts(e, counts) != SJCS_OK)
-        die_engine(e, "counts");
-    sjcs_engine_destroy(e);
-
-    /* result (main.c:660-664) */
+    /* result (main.c:660-664): rows come back while earlier rows are
+     * formatted */
     TIMER_START(tv);
-    if (sjcs_write_result(cli.outfile, &halos, counts, r2, cli.nrads, hthreads) < 0)
+    src.e = e;
+    src.slot = 1;
+    src.rows[0] = src.rows[1] = NULL;
+    if (sjcs_host_alloc(sizeof(uint64_t) * SJCS_RESULT_BATCH * (size_t)cli.nrads,
+                        (void **)&src.rows[0]) != SJCS_OK ||
+        sjcs_host_alloc(sizeof(uint64_t) * SJCS_RESULT_BATCH * (size_t)cli.nrads,
+                        (void **)&src.rows[1]) != SJCS_OK)
+        die_engine(NULL, "counts staging");
+    if (sjcs_write_result_rows(cli.outfile, &halos, fetch_rows, &src, r2, cli.nrads, hthreads) <
+        0)
         exit(EXIT_FAILURE);
     print_ms(">>>> result writing time", &tv);
+    trace("result written");
 
-    for (i = 0; i < 2; i++) {
-        free(bufs[i].xyz);
-        free(bufs[i].uv);
-    }
+    sjcs_host_free(src.rows[0]);
+    sjcs_host_free(src.rows[1]);
+    for (i = 0; i < 2; i++)
+        if (bufs[i].d)
+            sjcs_device_free(e, bufs[i].d);
+    sjcs_host_free(setup.pin[0]);
+    sjcs_host_free(setup.pin[1]);
+    sjcs_engine_destroy(e);
     sjcs_free_halos(&halos);
-    free(counts);
     free(rad);
     free(r2);
+    trace("exit");
     return 0;
 }

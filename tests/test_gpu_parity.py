@@ -675,6 +675,25 @@ def synth_snapshot(rng, side):
     return synth.gadget_snapshot(rng, side)
 
 
+def test_counts_range_follows_the_totals(eng):
+    """Row ranges equal the full read-out, and a range read after more
+    chunks (or a reset) sees the new totals, not a stale read-out."""
+    p, h = gen_uniform(31, 60_000, 5_000)
+    _, r2 = t.oracle_thresholds(0.01, 10.0, 12)
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    eng.add_particles(p[:30_000])
+    full = eng.counts()
+    parts = [eng.counts_range(a, b - a) for a, b in ((0, 1), (1, 2500), (2500, 4999), (4999, 5000))]
+    assert np.array_equal(np.vstack(parts), full)
+    assert eng.counts_range(5000, 0).shape == (0, 12)
+    eng.add_particles(p[30_000:])
+    want, _ = oracle_chunks([p], h, r2)
+    assert_counts_equal(eng.counts_range(0, 5000), want, h)
+    eng.reset_counts()
+    assert int(eng.counts_range(100, 50).sum()) == 0
+
+
 def test_full_size_config2_properties(eng):
     """Config 2 at its full size (1e8 particles x 1e7 halos x 40 shells),
     where the oracle is too slow: counts are independent of the cell size
