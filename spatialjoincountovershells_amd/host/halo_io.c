@@ -9,11 +9,16 @@
  *   - lines are read as fgets(buf, 16384) returns them, and the loop stops
  *     as soon as fgets reaches end-of-file, so a last line without '\n' is
  *     not a halo.
- * The whole file is read at once and its records are cut exactly where
- * those fgets calls would cut them, then parsed in parallel; strtoul and
- * strtof are the C library's, so every value is what the reference gets.
+ * The whole file is read at once (parallel preads) and its records are cut
+ * exactly where those fgets calls would cut them, then parsed in parallel.
+ * Every value is what the C library's strtoul / strtof give: plain decimal
+ * numbers take a fast path that is exact (below), anything else -- signs of
+ * strtoul, spaces, hex floats, inf, nan, 20 or more significant digits,
+ * large exponents, float subnormals and overflow, decimal float midpoints --
+ * goes to the library functions themselves.
  */
 #include <errno.h>
+#include <float.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,30 +28,122 @@
 
 #define REC_MAX (SJCS_HALO_LINE_BUF - 1) /* fgets reads at most size-1 chars */
 
+static const double k_pow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                   1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                   1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+#define IS_DIGIT(c) ((unsigned)((c) - '0') < 10u)
+
+/* strtof(s, NULL).  For [+-]digits[.digits][(e|E)[+-]digits] with at most
+ * 19 significant digits m and a decimal exponent |e| <= 22, m and 10^|e|
+ * are exact doubles, so m * 10^e (or m / 10^-e) is the decimal's value
+ * rounded once to double.  Float midpoints are doubles, and rounding is
+ * monotone, so that double and the decimal lie on the same side of every
+ * float midpoint unless the double is one: then its rounding to float is
+ * the decimal's correctly rounded float, as strtof's is.  A double on a
+ * midpoint, outside the normal float range, or any other form goes to
+ * strtof. */
+static float parse_float(const char *s)
+{
+    const char *p = s, *d0;
+    uint64_t m = 0, bits;
+    int neg = 0, nd = 0, frac = 0, any = 0, e10;
+    double q;
+    float f;
+    if (*p == '+' || *p == '-')
+        neg = *p++ == '-';
+    d0 = p;
+    for (; IS_DIGIT(*p); p++) {
+        any = 1;
+        if (m || *p != '0') {
+            if (nd == 19)
+                return strtof(s, NULL);
+            m = m * 10 + (uint64_t)(*p - '0');
+            nd++;
+        }
+    }
+    if (*p == '.') {
+        for (p++; IS_DIGIT(*p); p++) {
+            any = 1;
+            frac++;
+            if (m || *p != '0') {
+                if (nd == 19)
+                    return strtof(s, NULL);
+                m = m * 10 + (uint64_t)(*p - '0');
+                nd++;
+            }
+        }
+    }
+    /* "0x...", "inf", "nan", leading spaces, a lone "." */
+    if (!any || ((*p == 'x' || *p == 'X') && p == d0 + 1 && *d0 == '0'))
+        return strtof(s, NULL);
+    e10 = -frac;
+    if (*p == 'e' || *p == 'E') {
+        const char *q2 = p + 1;
+        int eneg = 0, ev = 0;
+        if (*q2 == '+' || *q2 == '-')
+            eneg = *q2++ == '-';
+        if (IS_DIGIT(*q2)) {
+            for (; IS_DIGIT(*q2); q2++)
+                if (ev < 100000)
+                    ev = ev * 10 + (*q2 - '0');
+            e10 += eneg ? -ev : ev;
+        }
+    }
+    if (m == 0)
+        return neg ? -0.0f : 0.0f;
+    if (e10 < -22 || e10 > 22)
+        return strtof(s, NULL);
+    q = e10 >= 0 ? (double)m * k_pow10[e10] : (double)m / k_pow10[-e10];
+    if (!(q >= (double)FLT_MIN && q <= (double)FLT_MAX))
+        return strtof(s, NULL);
+    memcpy(&bits, &q, sizeof(bits));
+    if ((bits & ((1ull << 29) - 1)) == (1ull << 28))
+        return strtof(s, NULL);
+    f = (float)q;
+    return neg ? -f : f;
+}
+
+/* strtoul(s, NULL, 10); digits-only values below 10^19 directly. */
+static uint64_t parse_u64(const char *s)
+{
+    const char *p = s;
+    uint64_t v = 0;
+    int nd = 0;
+    if (!IS_DIGIT(*p))
+        return strtoul(s, NULL, 10);
+    for (; IS_DIGIT(*p); p++) {
+        if (++nd > 19)
+            return strtoul(s, NULL, 10);
+        v = v * 10 + (uint64_t)(*p - '0');
+    }
+    return v;
+}
+
 int sjcs_parse_halo_line(const char *str, uint64_t *id, float *mass, float point[3])
 {
     int i;
-    *id = strtoul(str, NULL, 10);
+    *id = parse_u64(str);
     for (i = 0; i < 2; i++) {
         if ((str = strchr(str, ' ')) == NULL)
             return -1;
         str++;
     }
-    *mass = strtof(str, NULL);
+    *mass = parse_float(str);
     for (i = 0; i < 6; i++) {
         if ((str = strchr(str, ' ')) == NULL)
             return -1;
         str++;
     }
-    point[0] = strtof(str, NULL);
+    point[0] = parse_float(str);
     if ((str = strchr(str, ' ')) == NULL)
         return -1;
     str++;
-    point[1] = strtof(str, NULL);
+    point[1] = parse_float(str);
     if ((str = strchr(str, ' ')) == NULL)
         return -1;
     str++;
-    point[2] = strtof(str, NULL);
+    point[2] = parse_float(str);
     return 0;
 }
 
@@ -65,28 +162,22 @@ static size_t next_record(const char *buf, size_t size, size_t pos, int *at_eof)
     return lim;
 }
 
-static int slurp(const char *fname, char **out, size_t *size)
+static int slurp(const char *fname, char **out, size_t *size, int nthreads)
 {
-    FILE *fp = fopen(fname, "r");
     struct stat st;
     char *buf;
-    size_t got;
-    if (fp == NULL)
+    if (stat(fname, &st) != 0)
         return -1;
-    if (fstat(fileno(fp), &st) != 0) {
-        fclose(fp);
-        return -1;
-    }
     buf = malloc((size_t)st.st_size + 1);
-    if (buf == NULL) {
-        fclose(fp);
+    if (buf == NULL)
         return -2;
+    if (sjcs_pread_par(fname, 0, (uint64_t)st.st_size, buf, nthreads) != 0) {
+        free(buf);
+        return -1;
     }
-    got = fread(buf, 1, (size_t)st.st_size, fp);
-    fclose(fp);
-    buf[got] = '\0';
+    buf[st.st_size] = '\0';
     *out = buf;
-    *size = got;
+    *size = (size_t)st.st_size;
     return 0;
 }
 
@@ -150,7 +241,7 @@ static int parse_file(const char *fname, int nthreads, sjcs_halo **out, uint64_t
     uint64_t total = 0;
     int t, row, rc;
 
-    rc = slurp(fname, &buf, &size);
+    rc = slurp(fname, &buf, &size, nthreads);
     if (rc == -1)
         return -1;
     if (rc != 0)
@@ -264,7 +355,7 @@ static int read_filter(const char *fname, uint64_t **ids, uint64_t *n)
     size_t size = 0, pos = 0;
     uint64_t cnt = 0, i;
     char line[SJCS_HALO_LINE_BUF];
-    int rc = slurp(fname, &buf, &size);
+    int rc = slurp(fname, &buf, &size, 4);
     if (rc != 0)
         return rc;
     while (pos < size) {

@@ -284,8 +284,8 @@ int main(int argc, char *argv[])
     setup.first = argv[cli.particle_beg];
     setup.buf = &bufs[0];
     setup.compressed = cli.compressed;
-    /* a share of the threads: the halo parse runs meanwhile */
-    setup.nthreads = hthreads / 4 > 0 ? hthreads / 4 : 1;
+    /* half the threads: the halo parse runs meanwhile */
+    setup.nthreads = hthreads / 2 > 0 ? hthreads / 2 : 1;
     setup_threaded = pthread_create(&setup_th, NULL, setup_run, &setup) == 0;
     if (!setup_threaded)
         setup_run(&setup);

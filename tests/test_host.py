@@ -256,3 +256,65 @@ def test_gadget_decode_params_equal_oracle(built):
                 assert getattr(got, f) == want[f], f
     with pytest.raises(built.SJCSError):
         built.gadget_decode_params(gadget_header(1625 ** 3, 1000.0, 0.0, 1.0, 16, 1))
+
+
+def float_forms(rng):
+    """Decimal spellings a halo file may hold, and those that sit exactly on
+    a float midpoint or outside the fast path."""
+    out = []
+    f32 = rng.random(3000).astype(np.float32) * np.float32(1000.0)
+    wide = (rng.standard_normal(3000) * 10.0 ** rng.integers(-30, 30, 3000)).astype(np.float32)
+    for v in np.concatenate([f32, wide]):
+        v = float(v)
+        for fmt in ("%f", "%.9g", "%.17g", "%e", "%.3f", "%.12f", "%.20e"):
+            out.append(fmt % v)
+    # float midpoints written exactly: integers (ulp >= 2) and .5/.25 steps
+    for _ in range(2000):
+        e = int(rng.integers(21, 45))
+        m = int(rng.integers(1 << 23, 1 << 24))
+        ulp = 2 ** (e - 23)
+        mid = m * ulp + ulp / 2
+        out.append("%.3f" % mid if ulp < 2 else "%d" % int(mid))
+    # long mantissas, large exponents, special forms
+    for _ in range(2000):
+        digits = "".join(rng.choice(list("0123456789"), int(rng.integers(1, 26))))
+        dot = int(rng.integers(0, len(digits) + 1))
+        s = digits[:dot] + "." + digits[dot:]
+        if rng.random() < 0.5:
+            s += "e%d" % int(rng.integers(-45, 45))
+        out.append(("-" if rng.random() < 0.3 else "") + s)
+    out += ["-0.0", "+0", "-0e5", "0x1.2cp+8", "-0X1p-3", "inf", "-nan", " 5.5", ".5", "5.",
+            "1e-40", "1e-45", "3.4028235e38", "3.4028236e38", "3.5e38", "1e39", "1.5e", "2.5e+",
+            "7.25abc", "1.17549435e-38", "1.1754942e-38", "9999999999999999999.5",
+            "00000000000000000000000123.5", "16777217", "33554435", "0.000000000000000000001"]
+    return out
+
+
+def test_halo_line_numbers_equal_libc(host, tmp_path):
+    """Every coordinate and mass the halo parser returns equals the C
+    library's strtof of the same text, bit for bit, across plain decimals,
+    exact float midpoints, long mantissas, large exponents and the forms
+    that only the library handles; ids equal strtoul."""
+    rng = np.random.default_rng(5)
+    forms = float_forms(rng)
+    idv = ctypes.c_uint64()
+    mass = ctypes.c_float()
+    pt = (ctypes.c_float * 3)()
+    bad = []
+    for i, s in enumerate(forms):
+        a, b, c = s, forms[(i * 7 + 3) % len(forms)], forms[(i * 13 + 5) % len(forms)]
+        ids = ["%d" % (i * 7919), "18446744073709551615", "99999999999999999999", "-3", " 42"][i % 5]
+        line = f"{ids} 0 {b} 0 0 0 0 0 {a} {b} {c} 0\n".encode()
+        ref = t.parse_halo_line(line)
+        rc = host.sjcs_parse_halo_line(line, ctypes.byref(idv), ctypes.byref(mass), pt)
+        if ref is None:
+            assert rc == -1
+            continue
+        assert rc == 0
+        want = [ref[1]] + list(ref[2])
+        got = [mass.value, pt[0], pt[1], pt[2]]
+        wb = np.array(want, np.float32).view(np.uint32)
+        gb = np.array(got, np.float32).view(np.uint32)
+        if not np.array_equal(wb, gb) or idv.value != ref[0]:
+            bad.append((a, b, c, ids))
+    assert bad == [], bad[:5]
