@@ -484,6 +484,13 @@ def main():
             with open(tfile) as f:
                 decode["traffic"] = json.load(f).get(args.config, {}).get("decode")
 
+    # measured VALU issue of the count kernel (rocprofv3 counters, profiles/)
+    vfile = os.path.join(ROOT, "profiles", "valu.json")
+    valu = None
+    if os.path.exists(vfile):
+        with open(vfile) as f:
+            valu = json.load(f).get(args.config, {}).get("count")
+
     cpu = None
     if not args.no_cpu_baseline and args.config in ("c2", "c3", "c4", "c2z"):
         log("[bench] cpu baseline (reference astr_rcount on a halo sample)")
@@ -524,7 +531,10 @@ def main():
         "count_kernel": {"pair_tests_per_s": w / (count_ms * 1e-3),
                          "valu_tflops_8_per_test": round(count_tflops, 3),
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
-                         "hbm_gbs_algorithmic": round(count_gbs, 1)},
+                         "hbm_gbs_algorithmic": round(count_gbs, 1),
+                         "valu_busy_measured": valu and valu.get("valu_busy"),
+                         "valu_lane_slots_per_pair_test":
+                             valu and valu.get("valu_lane_slots_per_pair_test")},
         "autotune": tuned,
         "decode_kernel": decode,
         "wall": wall,
