@@ -734,3 +734,47 @@ def test_full_size_config2_properties(eng):
     sample = rng.choice(h.shape[0], 12, replace=False)
     want, _ = t.oracle_count(p, h[sample], r2, threads=16)
     assert np.array_equal(base[sample], want)
+
+
+def test_full_size_config3_properties(eng):
+    """The north-star scale: 3 particle files of 1e8 uniform particles
+    (distinct seeds) x 1e7 halos x 40 shells, counts accumulated on the
+    device.  The 3-file totals equal the sum of each file counted alone,
+    equal the graph-replayed chunk loop, equal the oracle on a sample of
+    halos, and W is near its expectation 3e9."""
+    rng = np.random.default_rng(3003)
+    h = rng.random((10_000_000, 3), dtype=np.float32) * np.float32(1000.0)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 40)
+    files = [np.random.default_rng(3100 + i).random((100_000_000, 3), dtype=np.float32) *
+             np.float32(1000.0) for i in range(3)]
+    ds = [eng.device_array(p) for p in files]
+    try:
+        eng.set_shells(r2)
+        eng.set_halos(h)
+        w = 0
+        each = []
+        for d in ds:
+            eng.reset_counts()
+            eng.add_particles_device(d, 100_000_000)
+            each.append(eng.counts())
+            w += eng.last_chunk_pair_tests()
+        print("config 3: W", w, flush=True)
+        assert abs(w / 3e9 - 1.0) < 0.005
+        eng.reset_counts()
+        for d in ds:
+            eng.add_particles_device(d, 100_000_000)
+        total = eng.counts()
+        assert np.array_equal(total, each[0] + each[1] + each[2])
+        eng.set_option("graph", 1)
+        eng.reset_counts()
+        for d in ds:
+            eng.add_particles_device(d, 100_000_000)
+        assert np.array_equal(eng.counts(), total)
+    finally:
+        eng.set_option("graph", 0)
+        for d in ds:
+            eng.device_free(d)
+    sample = rng.choice(h.shape[0], 8, replace=False)
+    want = sum(t.oracle_count(p, h[sample], r2, threads=16)[0] for p in files)
+    print("config 3: oracle sample done", flush=True)
+    assert np.array_equal(total[sample], want)
