@@ -907,7 +907,8 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
     if (h == hipSuccess) h = hipEventRecord(e->counted[i], e->stream);
     if (h == hipSuccess) h = hipEventRecord(e->uploaded[i], e->stream);
   }
-  if (h == hipSuccess) h = ensure(e->inbox, sizeof(unsigned long long));
+  // the pair-test total (u64) and the halo sort's run flag (u32)
+  if (h == hipSuccess) h = ensure(e->inbox, 16);
   if (h != hipSuccess) {
     g_create_error = std::string("engine setup: ") + hipGetErrorString(h);
     sjcs_engine_destroy(e);

@@ -297,6 +297,10 @@ int main(int argc, char *argv[])
     else
         rc = sjcs_read_and_filter_halos(cli.halo_file, cli.filter_file, &halos, err, sizeof(err));
     if (rc != 0) {
+        /* the device set-up finishes first: exiting under its runtime calls
+         * would race the runtime's teardown */
+        if (setup_threaded)
+            pthread_join(setup_th, NULL);
         fputs(err, stderr);
         exit(EXIT_FAILURE);
     }
@@ -307,14 +311,14 @@ int main(int argc, char *argv[])
     /* shells (main.c:645-649) */
     rad = malloc(sizeof(float) * (size_t)cli.nrads);
     r2 = malloc(sizeof(float) * (size_t)cli.nrads);
+    if (setup_threaded)
+        pthread_join(setup_th, NULL);
     if (!rad || !r2) {
         fprintf(stderr, "ERROR: memory shortage\n");
         exit(EXIT_FAILURE);
     }
     sjcs_host_shell_thresholds(cli.minrad, cli.maxrad, cli.nrads, rad, r2);
 
-    if (setup_threaded)
-        pthread_join(setup_th, NULL);
     e = setup.e;
     if (setup.rc != SJCS_OK)
         die_engine(e, "GPU engine");

@@ -101,6 +101,34 @@ def test_cli_golden_case(built, case, tmp_path):
     assert np.array_equal(eids, rids)
 
 
+@pytest.mark.parametrize("missing", ["particles", "halos", "second_particles", "compressed"])
+def test_cli_missing_inputs_with_device(built, missing, tmp_path):
+    """With the device set up on its own thread, a missing input still ends
+    the run with the reference's message and status 1 (no crash in the
+    runtime's teardown)."""
+    parts = ["uni_particles.dat"]
+    halo = "uni_halos.list"
+    extra = []
+    if missing == "particles":
+        parts = ["nope.dat"]
+    elif missing == "halos":
+        halo = "nope.list"
+    elif missing == "second_particles":
+        parts = ["uni_particles.dat", "nope.dat"]
+    else:
+        parts = ["nope.gdt"]
+        extra = ["--compressed"]
+    args = [built.CLI_PATH, "--thread", "4", "--next-setup", "2", "--particle-files"] + parts + \
+        extra + ["--halo-file", halo, "--radius", "0.5:5:4", "--output-file", str(tmp_path / "r")]
+    r = subprocess.run(args, cwd=t.GOLD, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stderr)
+    want = {"particles": "ERROR: file nope.dat not found in readDecodedParticles()",
+            "halos": "ERROR: file nope.list not found in readHalos().",
+            "second_particles": "ERROR: file nope.dat not found in readDecodedParticles()",
+            "compressed": "ERROR: file nope.gdt not found in readCompressedParticles()"}[missing]
+    assert want in r.stderr
+
+
 # ---------------------------------------------------------- engine parity --
 
 @pytest.mark.parametrize("seed", [1, 2])
