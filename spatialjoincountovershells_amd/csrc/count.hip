@@ -313,8 +313,10 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             for (int u = 0; u < SEG; ++u) {
               // a slot without a hit reads entry 0 (its distance may be
               // anything); a hit's bucket is below nlut
-              const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
-              e[u] = lut[hit[u] ? max(b, 0) : 0];
+              const uint32_t b =
+                  __builtin_elementwise_sub_sat(__float_as_uint(dist[u]), sh.lut_base) >>
+                  sh.lut_shift;
+              e[u] = lut[hit[u] ? b : 0u];
             }
 #pragma unroll
             for (int u = 0; u < SEG; ++u) {
