@@ -120,7 +120,7 @@ def write_halo_file(path, halos):
 
 def timer_lines(text):
     return {m.group(1): int(m.group(2))
-            for m in re.finditer(r">>>> ([a-z ]+) time: (\d+) milli seconds", text)}
+            for m in re.finditer(r">>>> ([a-z +]+) time: (\d+) milli seconds", text)}
 
 
 def is_compressed(part):
