@@ -34,9 +34,9 @@ static const double k_pow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7
 
 #define IS_DIGIT(c) ((unsigned)((c) - '0') < 10u)
 
-/* strtof(s, NULL).  For [+-]digits[.digits][(e|E)[+-]digits] with at most
- * 19 significant digits m and a decimal exponent |e| <= 22, m and 10^|e|
- * are exact doubles, so m * 10^e (or m / 10^-e) is the decimal's value
+/* strtof(s, NULL).  For [+-]digits[.digits][(e|E)[+-]digits] with a
+ * significand m < 2^53 (at most 19 digits read) and a decimal exponent
+ * |e| <= 22, m and 10^|e| are exact doubles, so m * 10^e (or m / 10^-e) is the decimal's value
  * rounded once to double.  Float midpoints are doubles, and rounding is
  * monotone, so that double and the decimal lie on the same side of every
  * float midpoint unless the double is one: then its rounding to float is
@@ -92,7 +92,7 @@ static float parse_float(const char *s)
     }
     if (m == 0)
         return neg ? -0.0f : 0.0f;
-    if (e10 < -22 || e10 > 22)
+    if ((m >> 53) != 0 || e10 < -22 || e10 > 22)
         return strtof(s, NULL);
     q = e10 >= 0 ? (double)m * k_pow10[e10] : (double)m / k_pow10[-e10];
     if (!(q >= (double)FLT_MIN && q <= (double)FLT_MAX))

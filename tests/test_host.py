@@ -283,6 +283,11 @@ def float_forms(rng):
         if rng.random() < 0.5:
             s += "e%d" % int(rng.integers(-45, 45))
         out.append(("-" if rng.random() < 0.3 else "") + s)
+    # significands around 2^53 (exact doubles stop there)
+    for _ in range(3000):
+        m = int(rng.integers(2 ** 52, 2 ** 60))
+        e = int(rng.integers(-22, 23))
+        out.append("%de%d" % (m, e))
     out += ["-0.0", "+0", "-0e5", "0x1.2cp+8", "-0X1p-3", "inf", "-nan", " 5.5", ".5", "5.",
             "1e-40", "1e-45", "3.4028235e38", "3.4028236e38", "3.5e38", "1e39", "1.5e", "2.5e+",
             "7.25abc", "1.17549435e-38", "1.1754942e-38", "9999999999999999999.5",
