@@ -288,6 +288,10 @@ def float_forms(rng):
         m = int(rng.integers(2 ** 52, 2 ** 60))
         e = int(rng.integers(-22, 23))
         out.append("%de%d" % (m, e))
+    # significands above 2^53 next to float midpoints: rounding the
+    # significand to double first would land on the wrong side
+    out += ["606177124023437500e-14", "606177124023437501e-14", "606177124023437503e-14",
+            "6033258007812499997e-14", "6033258007812499999e-14", "6033258007812500000e-14"]
     out += ["-0.0", "+0", "-0e5", "0x1.2cp+8", "-0X1p-3", "inf", "-nan", " 5.5", ".5", "5.",
             "1e-40", "1e-45", "3.4028235e38", "3.4028236e38", "3.5e38", "1e39", "1.5e", "2.5e+",
             "7.25abc", "1.17549435e-38", "1.1754942e-38", "9999999999999999999.5",
