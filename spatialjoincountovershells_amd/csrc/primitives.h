@@ -29,8 +29,8 @@ constexpr int kCsThreads = 512;
 // staging takes 96 KB of LDS
 constexpr int kCsItems = 16;
 constexpr int kCsTile = kCsThreads * kCsItems;
-constexpr int kCsRowThreads = 256;
-constexpr int kCsRowItems = 16;
+constexpr int kCsRowThreads = 512;
+constexpr int kCsRowItems = 8;
 constexpr int kCsRowCap = kCsRowThreads * kCsRowItems;  // 4096 particles
 constexpr int kCsMaxCells = 256;    // along y and z
 constexpr int kCsMaxCellsX = 1024;  // along x
