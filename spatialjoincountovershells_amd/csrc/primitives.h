@@ -20,8 +20,9 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Cell sort (grids of at most kCsMaxCells cells per axis): tiles of the
-// two scatter passes, and rows of the last pass staged in LDS.
+// Cell sort (grids of at most kCsMaxCells cells along y and z,
+// kCsMaxCellsX along x): tiles of the scatter passes, and rows of up to
+// kCsRowCap particles of the last pass sorted in LDS.
 constexpr int kCsThreads = 512;
 // 8192 particles per block: each digit's run per tile is ~37 particles, so
 // few cache lines are written in part (the scatter passes write 1.02x the
@@ -45,12 +46,32 @@ inline uint64_t radix_workspace_words(uint64_t n) {
   return hist + ceil_div(hist, kScanTile) + 64;
 }
 
-// Workspace (u32 words) of cell_sort_particles for n particles.
+// Rows of the last cell-sort pass longer than kCsRowCap particles are cut
+// into kCsTile tiles: at most n / (kCsRowCap + 1) such rows (and at most one
+// per row of the grid), and at most one partial tile each.
+inline uint64_t cs_long_rows_max(uint64_t n) {
+  const uint64_t a = n / (kCsRowCap + 1), b = (uint64_t)kCsMaxCells * kCsMaxCells;
+  return a < b ? a : b;
+}
+inline uint64_t cs_long_tiles_max(uint64_t n) { return ceil_div(n, kCsTile) + cs_long_rows_max(n); }
+
+// Words of the cell sort's tile histograms: those of the z and y passes,
+// later those of the long rows (kCsMaxCellsX digits per tile) in the same
+// words.
+inline uint64_t cs_hist_words(uint64_t n) {
+  const uint64_t tiles1 = ceil_div(n ? n : 1, kCsTile);
+  uint64_t hist = (uint64_t)kCsMaxCells * (2 * tiles1 + kCsMaxCells);
+  const uint64_t hist3 = (uint64_t)kCsMaxCellsX * cs_long_tiles_max(n);
+  if (hist3 > hist) hist = hist3;
+  return (hist + 3) & ~3ull;
+}
+// Workspace (u32 words) of cell_sort_particles for n particles: the tile
+// histograms, the long rows' records (4 words each) and their tiles' rows,
+// slab starts and tiles, two counters and the scan partials.
 inline uint64_t cell_sort_workspace_words(uint64_t n) {
-  uint64_t tiles1 = ceil_div(n ? n : 1, kCsTile);
-  uint64_t tiles2 = tiles1 + kCsMaxCells;
-  uint64_t hist = (uint64_t)kCsMaxCells * (tiles1 + tiles2);
-  return hist + 2 * (kCsMaxCells + 1) + ceil_div((uint64_t)kCsMaxCells * tiles2, kScanTile) + 64;
+  const uint64_t tiles2 = ceil_div(n ? n : 1, kCsTile) + kCsMaxCells;
+  return cs_hist_words(n) + 4 * cs_long_rows_max(n) + cs_long_tiles_max(n) +
+         2 * (kCsMaxCells + 1) + 4 + ceil_div((uint64_t)kCsMaxCells * tiles2, kScanTile) + 64;
 }
 
 // Workspace (u32 words) for a scan of n words.

@@ -376,17 +376,63 @@ __device__ __forceinline__ bool tile_range_slab(uint32_t t, const uint32_t* ss,
   return true;
 }
 
-// Per-tile histogram of the cells of coordinate c[i * stride].
-template <bool SLAB>
-__global__ __launch_bounds__(kCsThreads) void cs_hist_kernel(
-    const float* __restrict__ c, uint32_t stride, uint64_t n, float inv, int G, uint32_t ntiles,
-    const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_h[kCsMaxCells];
+// LDS histogram count of digit d for the lanes with `valid`.  The lanes that
+// hold the digit of the wave's first valid lane add in one atomic: clustered
+// particles put many lanes of a wave on one digit, and LDS atomics on one
+// address serialise.  Called by the whole wave.
+__device__ __forceinline__ void lds_count(uint32_t* cnt, uint32_t d, bool valid) {
+  const uint64_t vm = __ballot(valid);
+  if (!vm) return;
+  const int first = __builtin_ctzll(vm);
+  const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, first);
+  const uint64_t peers = __ballot(valid && d == d0);
+  const bool lead = lane_id() == first;
+  if (valid && (d != d0 || lead)) atomicAdd(&cnt[d], lead ? (uint32_t)__popcll(peers) : 1u);
+}
+
+// Long row k of the last pass: lrows[k] = (row cz * G + cy, first tile,
+// first particle, length); tile t belongs to long row trow[t].  The row's
+// tiles are kCsTile particles (the last one shorter); their cx histograms
+// are tile-major: hist[(first tile + i) * GX + d].
+struct LongTile {
+  uint32_t row, tile0, start, len, i, ts;
+};
+
+__device__ __forceinline__ LongTile long_tile(uint32_t t, const uint4* __restrict__ lrows,
+                                              const uint32_t* __restrict__ trow) {
+  const uint4 L = lrows[trow[t]];
+  return LongTile{L.x, L.y, L.z, L.w, t - L.y, (L.w + kCsTile - 1) / kCsTile};
+}
+
+__device__ __forceinline__ TileRange long_range(const LongTile& lt, int GX) {
   TileRange r;
-  const uint32_t t = xcd_tile();
-  if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
+  r.begin = (uint64_t)lt.start + (uint64_t)lt.i * kCsTile;
+  const uint32_t left = lt.len - lt.i * (uint32_t)kCsTile;
+  r.len = left < (uint32_t)kCsTile ? left : (uint32_t)kCsTile;
+  r.hbase = (uint64_t)(lt.tile0 + lt.i) * (uint64_t)GX;
+  r.hstride = 1;
+  return r;
+}
+
+// Tiles [0, ntiles) of a persistent grid of a multiple of 8 blocks: XCD x
+// takes the contiguous range [ntiles * x / 8, ntiles * (x + 1) / 8), its
+// blocks in turn, so neighbouring tiles meet in one L2.
+__device__ __forceinline__ void xcd_range(uint32_t ntiles, uint32_t& t, uint32_t& end,
+                                          uint32_t& step) {
+  const uint32_t x = blockIdx.x & 7u;
+  step = gridDim.x >> 3;
+  t = (uint32_t)((uint64_t)ntiles * x / 8) + (blockIdx.x >> 3);
+  end = (uint32_t)((uint64_t)ntiles * (x + 1) / 8);
+}
+
+// Histogram of tile r's cells of coordinate c[i * stride] (ndig <= kDig
+// cells, 1/inv wide), written to hist[r.hbase + d * r.hstride].
+template <int kDig>
+__device__ __forceinline__ void hist_tile(const float* __restrict__ c, uint32_t stride, float inv,
+                                          int ndig, const TileRange& r, uint32_t* s_h,
+                                          uint32_t* __restrict__ hist) {
   const int tid = threadIdx.x;
-  for (int i = tid; i < kCsMaxCells; i += kCsThreads) s_h[i] = 0;
+  for (int i = tid; i < kDig; i += kCsThreads) s_h[i] = 0;
   __syncthreads();
   const float* cb = c + r.begin * stride;
   float v[kCsItems];
@@ -396,35 +442,49 @@ __global__ __launch_bounds__(kCsThreads) void cs_hist_kernel(
     v[j] = i < r.len ? cb[(uint64_t)i * stride] : 0.0f;
   }
 #pragma unroll
-  for (int j = 0; j < kCsItems; ++j)
-    if ((uint32_t)(j * kCsThreads + tid) < r.len) atomicAdd(&s_h[cell_of(v[j], inv, G - 1)], 1u);
+  for (int j = 0; j < kCsItems; ++j) {
+    const bool ok = (uint32_t)(j * kCsThreads + tid) < r.len;
+    lds_count(s_h, ok ? (uint32_t)cell_of(v[j], inv, ndig - 1) : 0u, ok);
+  }
   __syncthreads();
-  for (int d = tid; d < G; d += kCsThreads) hist[r.hbase + (uint64_t)d * r.hstride] = s_h[d];
+  for (int d = tid; d < ndig; d += kCsThreads) hist[r.hbase + (uint64_t)d * r.hstride] = s_h[d];
+}
+
+// Per-tile histogram of the cells of coordinate c[i * stride] (passes 1, 2).
+template <bool SLAB>
+__global__ __launch_bounds__(kCsThreads) void cs_hist_kernel(
+    const float* __restrict__ c, uint32_t stride, uint64_t n, float inv, int G, uint32_t ntiles,
+    const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kCsMaxCells];
+  TileRange r;
+  const uint32_t t = xcd_tile();
+  if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
+  hist_tile<kCsMaxCells>(c, stride, inv, G, r, s_h, hist);
 }
 
 struct Packed3 {
   float x, y, z;
 };
 
-// Scatter of one tile by the cell of coordinate AXIS to the offsets of the
-// scanned histograms.  Input packed xyz (AOS) or SoA; output SoA.
-template <int AXIS, bool AOS, bool SLAB>
-__global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
+// Scatter of tile r by the cell of coordinate AXIS (ndig <= kDig cells, 1/inv
+// wide).  Without `lt` the digit runs go to the offsets of the scanned
+// histograms hist[r.hbase + d * r.hstride].  With `lt` (a long row of pass
+// 3) the offsets come from the row's tile-major counts: the row's start,
+// plus the row's particles of lower digits, plus the digit's particles in
+// the row's earlier tiles; the row's first tile also writes the row's cell
+// table.  Input packed xyz (AOS) or SoA; output SoA.
+template <int AXIS, bool AOS, int kDig>
+__device__ __forceinline__ void scatter_tile(
     const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
-    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, uint64_t n, Grid g,
-    uint32_t ntiles, const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb,
-    const uint32_t* __restrict__ hist) {
-  __shared__ float s_c[3][kCsTile];
-  __shared__ uint32_t s_cnt[kCsMaxCells];
-  __shared__ uint32_t s_off[kCsMaxCells];
-  __shared__ uint32_t s_tmp[kCsThreads / 64];
-  const int G = g.g, gm1 = G - 1;
-  const float inv = g.inv_cell;
-  TileRange r;
-  const uint32_t t = xcd_tile();
-  if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
+    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float inv, int ndig,
+    const TileRange& r, const uint32_t* __restrict__ hist, const LongTile* lt,
+    uint32_t* __restrict__ cell_end, float (*s_c)[kCsTile], uint32_t* s_cnt, uint32_t* s_off,
+    uint32_t* s_tmp) {
+  static_assert(kDig <= kCsThreads || kDig % kCsThreads == 0, "digits per thread");
+  constexpr int Q = kDig > kCsThreads ? kDig / kCsThreads : 1;
+  const int dm1 = ndig - 1;
   const int tid = threadIdx.x;
-  for (int i = tid; i < kCsMaxCells; i += kCsThreads) s_cnt[i] = 0;
+  for (int i = tid; i < kDig; i += kCsThreads) s_cnt[i] = 0;
 
   float px[kCsItems], py[kCsItems], pz[kCsItems];
 #pragma unroll
@@ -446,25 +506,71 @@ __global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
   }
   __syncthreads();
 
-  uint32_t rk[kCsItems];  // digit << 16 | rank among the tile's items of that digit
+  uint32_t rk[kCsItems];  // digit << 16 | slot among the tile's items of that digit
 #pragma unroll
   for (int j = 0; j < kCsItems; ++j) {
     rk[j] = 0;
     if ((uint32_t)(j * kCsThreads + tid) < r.len) {
       const float v = AXIS == 0 ? px[j] : (AXIS == 1 ? py[j] : pz[j]);
-      const uint32_t d = (uint32_t)cell_of(v, inv, gm1);
+      const uint32_t d = (uint32_t)cell_of(v, inv, dm1);
       rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
     }
   }
   __syncthreads();
-  const uint32_t c = tid < kCsMaxCells ? s_cnt[tid] : 0u;
+  uint32_t c[Q], sum = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int d = tid * Q + q;
+    c[q] = d < kDig ? s_cnt[d] : 0u;
+    sum += c[q];
+  }
   uint32_t total;
-  const uint32_t ls = block_excl_scan<OpSum, kCsThreads>(c, s_tmp, &total);
-  if (tid < G) {
-    // out index = offset of the tile's digit run + (slot - local start);
-    // u32 arithmetic wraps back into range
-    s_off[tid] = hist[r.hbase + (uint64_t)tid * r.hstride] - ls;
-    s_cnt[tid] = ls;
+  uint32_t ls = block_excl_scan<OpSum, kCsThreads>(sum, s_tmp, &total);
+  if (lt) {
+    // the digit's count in the row's earlier tiles (pre) and in all of them
+    // (tot); consecutive threads read consecutive digits of each tile
+    uint32_t pre[Q], tot[Q], tsum = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int d = tid * Q + q;
+      pre[q] = tot[q] = 0u;
+      if (d < ndig) {
+        const uint32_t* hc = hist + (uint64_t)lt->tile0 * (uint64_t)ndig + (uint64_t)d;
+        uint32_t i = 0;
+#pragma unroll 4
+        for (; i < lt->i; ++i) pre[q] += hc[(uint64_t)i * ndig];
+        tot[q] = pre[q];
+#pragma unroll 4
+        for (; i < lt->ts; ++i) tot[q] += hc[(uint64_t)i * ndig];
+      }
+      tsum += tot[q];
+    }
+    uint32_t rtotal;
+    uint32_t ds = lt->start + block_excl_scan<OpSum, kCsThreads>(tsum, s_tmp, &rtotal);
+    uint32_t* ce = cell_end + (uint64_t)lt->row * (uint64_t)ndig;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int d = tid * Q + q;
+      if (d < ndig) {
+        s_off[d] = ds + pre[q] - ls;
+        s_cnt[d] = ls;
+        if (lt->i == 0) ce[d] = ds + tot[q];
+      }
+      ds += tot[q];
+      ls += c[q];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int d = tid * Q + q;
+      if (d < ndig) {
+        // out index = offset of the tile's digit run + (slot - local start);
+        // u32 arithmetic wraps back into range
+        s_off[d] = hist[r.hbase + (uint64_t)d * r.hstride] - ls;
+        s_cnt[d] = ls;
+      }
+      ls += c[q];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -480,18 +586,38 @@ __global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
   for (uint32_t l = tid; l < r.len; l += kCsThreads) {
     const float x = s_c[0][l], y = s_c[1][l], z = s_c[2][l];
     const float v = AXIS == 0 ? x : (AXIS == 1 ? y : z);
-    const uint32_t gp = s_off[cell_of(v, inv, gm1)] + l;
+    const uint32_t gp = s_off[cell_of(v, inv, dm1)] + l;
     ox[gp] = x;
     oy[gp] = y;
     oz[gp] = z;
   }
 }
 
+// Scatter of one tile of passes 1 (by cz, chunk tiles) and 2 (by cy, slab
+// tiles).
+template <int AXIS, bool AOS, bool SLAB>
+__global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
+    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, uint64_t n, Grid g,
+    uint32_t ntiles, const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb,
+    const uint32_t* __restrict__ hist) {
+  __shared__ float s_c[3][kCsTile];
+  __shared__ uint32_t s_cnt[kCsMaxCells];
+  __shared__ uint32_t s_off[kCsMaxCells];
+  __shared__ uint32_t s_tmp[kCsThreads / 64];
+  TileRange r;
+  const uint32_t t = xcd_tile();
+  if (!(SLAB ? tile_range_slab(t, ss, tb, g.g, r) : tile_range_chunk(t, ntiles, n, r))) return;
+  scatter_tile<AXIS, AOS, kCsMaxCells>(ix, iy, iz, ox, oy, oz, g.inv_cell, g.g, r, hist, nullptr,
+                                       nullptr, s_c, s_cnt, s_off, s_tmp);
+}
+
 // Slab starts and the slab-major tiling of pass 2, from pass 1's scanned
-// histograms.  One block of kCsMaxCells threads.
+// histograms; clears the long-row counters of pass 3.  One block of
+// kCsMaxCells threads.
 __global__ __launch_bounds__(kCsMaxCells) void cs_slab_tiles_kernel(
     const uint32_t* __restrict__ hist1, uint32_t tiles1, uint64_t n, int G,
-    uint32_t* __restrict__ ss, uint32_t* __restrict__ tb) {
+    uint32_t* __restrict__ ss, uint32_t* __restrict__ tb, uint32_t* __restrict__ lctr) {
   __shared__ uint32_t s_tmp[kCsMaxCells / 64];
   const int tid = threadIdx.x;
   const uint32_t st = tid < G ? hist1[(uint64_t)tid * tiles1] : 0u;
@@ -507,6 +633,7 @@ __global__ __launch_bounds__(kCsMaxCells) void cs_slab_tiles_kernel(
     ss[G] = (uint32_t)n;
     tb[G] = total;
   }
+  if (tid < 2) lctr[tid] = 0u;
 }
 
 // First particle of row rw = cz * G + cy after pass 2 (rw = G*G: n).
@@ -522,9 +649,9 @@ __device__ __forceinline__ uint32_t row_begin(uint32_t rw, int G, uint64_t n,
 // Exclusive scan of the row's cx counts s_cnt[0 .. kCsMaxCellsX) with
 // kCsRowThreads threads, kCsMaxCellsX / kCsRowThreads consecutive digits
 // each.  Writes the row's cell table entries (start + inclusive count) and
-// leaves in s_cnt[d] the digit's first slot, plus `add`.
+// leaves in s_cnt[d] the digit's first slot.
 __device__ __forceinline__ void cs_row_scan(uint32_t* s_cnt, uint32_t* s_tmp, uint32_t* ce,
-                                            uint32_t start, int gx, uint32_t add) {
+                                            uint32_t start, int gx) {
   constexpr int Q = kCsMaxCellsX / kCsRowThreads;
   const int tid = threadIdx.x;
   uint32_t c[Q], sum = 0;
@@ -540,20 +667,22 @@ __device__ __forceinline__ void cs_row_scan(uint32_t* s_cnt, uint32_t* s_tmp, ui
     const int d = tid * Q + q;
     if (d < gx) {
       ce[d] = start + run + c[q];
-      s_cnt[d] = add + run;
+      s_cnt[d] = run;
     }
     run += c[q];
   }
 }
 
 // Pass 3: every row in cx order, and the cell table.  Rows of up to
-// kCsRowCap particles are staged in LDS and written out contiguously;
-// longer ones are counted and then scattered straight from HBM.
+// kCsRowCap particles are sorted in LDS and written out contiguously, one
+// block each.  A longer row is entered in the long-row list (lctr[0] counts
+// tiles, lctr[1] rows) for the tiled kernels below.
 __global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
     const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
     float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
     uint32_t* __restrict__ cell_end, uint64_t n, Grid g, const uint32_t* __restrict__ ss,
-    const uint32_t* __restrict__ tb, const uint32_t* __restrict__ hist2, uint32_t nrows) {
+    const uint32_t* __restrict__ tb, const uint32_t* __restrict__ hist2, uint32_t nrows,
+    uint32_t* __restrict__ lctr, uint4* __restrict__ lrows, uint32_t* __restrict__ trow) {
   __shared__ float s_c[3][kCsRowCap];
   __shared__ uint32_t s_cnt[kCsMaxCellsX];
   __shared__ uint32_t s_tmp[kCsRowThreads / 64];
@@ -564,62 +693,99 @@ __global__ __launch_bounds__(kCsRowThreads) void cs_row_kernel(
   const int tid = threadIdx.x;
   const uint32_t start = row_begin(rw, G, n, ss, tb, hist2);
   const uint32_t len = row_begin(rw + 1, G, n, ss, tb, hist2) - start;
+  if (len > (uint32_t)kCsRowCap) {
+    const uint32_t nt = (len + kCsTile - 1) / kCsTile;
+    if (tid == 0) {
+      const uint32_t first = atomicAdd(&lctr[0], nt);
+      const uint32_t k = atomicAdd(&lctr[1], 1u);
+      lrows[k] = make_uint4(rw, first, start, len);
+      s_tmp[0] = first;
+      s_tmp[1] = k;
+    }
+    __syncthreads();
+    const uint32_t first = s_tmp[0], k = s_tmp[1];
+    for (uint32_t i = tid; i < nt; i += kCsRowThreads) trow[first + i] = k;
+    return;
+  }
   for (int i = tid; i < kCsMaxCellsX; i += kCsRowThreads) s_cnt[i] = 0;
+  float px[kCsRowItems], py[kCsRowItems], pz[kCsRowItems];
+#pragma unroll
+  for (int j = 0; j < kCsRowItems; ++j) {
+    const uint32_t i = (uint32_t)(j * kCsRowThreads + tid);
+    px[j] = py[j] = pz[j] = 0.0f;
+    if (i < len) {
+      px[j] = ix[start + i];
+      py[j] = iy[start + i];
+      pz[j] = iz[start + i];
+    }
+  }
   __syncthreads();
   uint32_t* ce = cell_end + (uint64_t)rw * (uint64_t)GX;
+  uint32_t rk[kCsRowItems];  // digit << 16 | slot among the row's items of that digit
+#pragma unroll
+  for (int j = 0; j < kCsRowItems; ++j) {
+    rk[j] = 0;
+    if ((uint32_t)(j * kCsRowThreads + tid) < len) {
+      const uint32_t d = (uint32_t)cell_of(px[j], invx, gxm1);
+      rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
+    }
+  }
+  __syncthreads();
+  cs_row_scan(s_cnt, s_tmp, ce, start, GX);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kCsRowItems; ++j) {
+    if ((uint32_t)(j * kCsRowThreads + tid) < len) {
+      const uint32_t lp = s_cnt[rk[j] >> 16] + (rk[j] & 0xffffu);
+      s_c[0][lp] = px[j];
+      s_c[1][lp] = py[j];
+      s_c[2][lp] = pz[j];
+    }
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < len; l += kCsRowThreads) {
+    ox[start + l] = s_c[0][l];
+    oy[start + l] = s_c[1][l];
+    oz[start + l] = s_c[2][l];
+  }
+}
 
-  if (len <= (uint32_t)kCsRowCap) {
-    float px[kCsRowItems], py[kCsRowItems], pz[kCsRowItems];
-    uint32_t rk[kCsRowItems];
-#pragma unroll
-    for (int j = 0; j < kCsRowItems; ++j) {
-      const uint32_t i = (uint32_t)(j * kCsRowThreads + tid);
-      px[j] = py[j] = pz[j] = 0.0f;
-      if (i < len) {
-        px[j] = ix[start + i];
-        py[j] = iy[start + i];
-        pz[j] = iz[start + i];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kCsRowItems; ++j) {
-      rk[j] = 0;
-      if ((uint32_t)(j * kCsRowThreads + tid) < len) {
-        const uint32_t d = (uint32_t)cell_of(px[j], invx, gxm1);
-        rk[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
-      }
-    }
-    __syncthreads();
-    cs_row_scan(s_cnt, s_tmp, ce, start, GX, 0u);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kCsRowItems; ++j) {
-      if ((uint32_t)(j * kCsRowThreads + tid) < len) {
-        const uint32_t lp = s_cnt[rk[j] >> 16] + (rk[j] & 0xffffu);
-        s_c[0][lp] = px[j];
-        s_c[1][lp] = py[j];
-        s_c[2][lp] = pz[j];
-      }
-    }
-    __syncthreads();
-    for (uint32_t l = tid; l < len; l += kCsRowThreads) {
-      ox[start + l] = s_c[0][l];
-      oy[start + l] = s_c[1][l];
-      oz[start + l] = s_c[2][l];
-    }
-  } else {
-    for (uint32_t l = tid; l < len; l += kCsRowThreads)
-      atomicAdd(&s_cnt[cell_of(ix[start + l], invx, gxm1)], 1u);
-    __syncthreads();
-    cs_row_scan(s_cnt, s_tmp, ce, start, GX, start);  // s_cnt: output cursor of every cell
-    __syncthreads();
-    for (uint32_t l = tid; l < len; l += kCsRowThreads) {
-      const float x = ix[start + l], y = iy[start + l], z = iz[start + l];
-      const uint32_t p = atomicAdd(&s_cnt[cell_of(x, invx, gxm1)], 1u);
-      ox[p] = x;
-      oy[p] = y;
-      oz[p] = z;
-    }
+// Pass 3, long rows: per-tile cx histograms of every long row's tiles
+// (persistent grid over the lctr[0] tiles).
+__global__ __launch_bounds__(kCsThreads) void cs_long_hist_kernel(
+    const float* __restrict__ x, Grid g, const uint32_t* __restrict__ lctr,
+    const uint4* __restrict__ lrows, const uint32_t* __restrict__ trow,
+    uint32_t* __restrict__ hist3) {
+  __shared__ uint32_t s_h[kCsMaxCellsX];
+  uint32_t t, end, step;
+  xcd_range(lctr[0], t, end, step);
+  for (; t < end; t += step) {
+    const TileRange r = long_range(long_tile(t, lrows, trow), g.gx);
+    __syncthreads();  // the previous tile's readers of s_h are done
+    hist_tile<kCsMaxCellsX>(x, 1u, g.inv_cell_x, g.gx, r, s_h, hist3);
+  }
+}
+
+// Pass 3, long rows: every long row's tiles scattered by cx, and the long
+// rows' cell table (persistent grid over the lctr[0] tiles).
+__global__ __launch_bounds__(kCsThreads) void cs_long_scatter_kernel(
+    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, Grid g,
+    const uint32_t* __restrict__ lctr, const uint4* __restrict__ lrows,
+    const uint32_t* __restrict__ trow, const uint32_t* __restrict__ hist3,
+    uint32_t* __restrict__ cell_end) {
+  __shared__ float s_c[3][kCsTile];
+  __shared__ uint32_t s_cnt[kCsMaxCellsX];
+  __shared__ uint32_t s_off[kCsMaxCellsX];
+  __shared__ uint32_t s_tmp[kCsThreads / 64];
+  uint32_t t, end, step;
+  xcd_range(lctr[0], t, end, step);
+  for (; t < end; t += step) {
+    const LongTile lt = long_tile(t, lrows, trow);
+    const TileRange r = long_range(lt, g.gx);
+    __syncthreads();  // the previous tile's readers of the LDS are done
+    scatter_tile<0, false, kCsMaxCellsX>(ix, iy, iz, ox, oy, oz, g.inv_cell_x, g.gx, r, hist3, &lt,
+                                         cell_end, s_c, s_cnt, s_off, s_tmp);
   }
 }
 
@@ -750,11 +916,16 @@ hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2]
   const uint64_t tiles1 = ceil_div(n, kCsTile);
   const uint64_t tiles2 = tiles1 + (uint64_t)G;  // each slab adds at most one partial tile
   if (tiles2 * (uint64_t)G > 0xffffffffull) return hipErrorInvalidValue;
+  // the long rows' histograms (pass 3) reuse the words of passes 1 and 2
   uint32_t* hist1 = ws;
   uint32_t* hist2 = hist1 + (uint64_t)kCsMaxCells * tiles1;
-  uint32_t* ss = hist2 + (uint64_t)kCsMaxCells * tiles2;
+  uint32_t* hist3 = ws;
+  uint4* lrows = reinterpret_cast<uint4*>(ws + cs_hist_words(n));
+  uint32_t* trow = reinterpret_cast<uint32_t*>(lrows + cs_long_rows_max(n));
+  uint32_t* ss = trow + cs_long_tiles_max(n);
   uint32_t* tb = ss + kCsMaxCells + 1;
-  uint32_t* scan_ws = tb + kCsMaxCells + 1;
+  uint32_t* lctr = tb + kCsMaxCells + 1;
+  uint32_t* scan_ws = lctr + 4;
   const uint32_t nt1 = (uint32_t)tiles1;
   const uint32_t nb1 = (uint32_t)(8 * ceil_div(tiles1, 8));
   const uint32_t nb2 = (uint32_t)(8 * ceil_div(tiles2, 8));
@@ -767,16 +938,27 @@ hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2]
   if (e != hipSuccess) return e;
   cs_scatter_kernel<2, true, false><<<nb1, kCsThreads, 0, s>>>(
       xyz, nullptr, nullptr, x[0], y[0], z[0], n, g, nt1, nullptr, nullptr, hist1);
-  cs_slab_tiles_kernel<<<1, kCsMaxCells, 0, s>>>(hist1, nt1, n, G, ss, tb);
+  cs_slab_tiles_kernel<<<1, kCsMaxCells, 0, s>>>(hist1, nt1, n, G, ss, tb, lctr);
   // pass 2: by cy inside each slab, buffers 0 -> 1
   cs_hist_kernel<true><<<nb2, kCsThreads, 0, s>>>(y[0], 1u, n, g.inv_cell, G, 0u, ss, tb, hist2);
   e = exclusive_sum_u32(hist2, hist2, (uint64_t)G * tiles2, scan_ws, s);
   if (e != hipSuccess) return e;
   cs_scatter_kernel<1, false, true><<<nb2, kCsThreads, 0, s>>>(
       x[0], y[0], z[0], x[1], y[1], z[1], n, g, 0u, ss, tb, hist2);
-  // pass 3: by cx inside each row, buffers 1 -> 0, and the cell table
+  // pass 3: by cx inside each row, buffers 1 -> 0, and the cell table; rows
+  // of more than kCsRowCap particles in kCsTile tiles, like passes 1 and 2
   cs_row_kernel<<<nb3, kCsRowThreads, 0, s>>>(x[1], y[1], z[1], x[0], y[0], z[0], cell_end, n, g,
-                                              ss, tb, hist2, nrows);
+                                              ss, tb, hist2, nrows, lctr, lrows, trow);
+  const uint64_t long_rows = cs_long_rows_max(n) < nrows ? cs_long_rows_max(n) : nrows;
+  if (long_rows > 0) {
+    // persistent grids: the scatter's LDS allows one block per CU
+    const uint64_t per_xcd = ceil_div(cs_long_tiles_max(n), 8);
+    const uint32_t nbh = (uint32_t)(8 * (per_xcd < 64 ? per_xcd : 64));
+    const uint32_t nbc = (uint32_t)(8 * (per_xcd < 32 ? per_xcd : 32));
+    cs_long_hist_kernel<<<nbh, kCsThreads, 0, s>>>(x[1], g, lctr, lrows, trow, hist3);
+    cs_long_scatter_kernel<<<nbc, kCsThreads, 0, s>>>(x[1], y[1], z[1], x[0], y[0], z[0], g, lctr,
+                                                       lrows, trow, hist3, cell_end);
+  }
   return hipGetLastError();
 }
 

@@ -297,6 +297,65 @@ def test_clustered_particles(eng, index_sort):
     assert_counts_equal(got, want, h)
 
 
+def long_rows_input(seed, lengths, clumped):
+    """Uniform background plus rows of the 64 x 64 (y, z) cell grid (15.625
+    wide) holding exactly `lengths` particles; rows in `clumped` put all
+    their added particles in one x cell (500.1 .. 500.4)."""
+    G = 64
+    cs = 1000.0 / G
+    inv = np.float32(G) / np.float32(1000.0)
+    rng = np.random.default_rng(seed)
+
+    def row_of(q):
+        c = np.clip(np.floor(q[:, 1:] * inv), 0, G - 1).astype(np.int64)
+        return c[:, 1] * G + c[:, 0]
+
+    bg = rng.random((20_000, 3), dtype=np.float32) * np.float32(1000.0)
+    rows = rng.choice(G * G, len(lengths), replace=False)
+    parts, halos = [bg], [rng.random((100, 3), dtype=np.float32) * np.float32(1000.0)]
+    have = np.bincount(row_of(bg), minlength=G * G)
+    for i, (rw, ln) in enumerate(zip(rows, lengths)):
+        cz, cy = divmod(int(rw), G)
+        m = ln - int(have[rw])
+        q = np.empty((m, 3), np.float32)
+        q[:, 0] = (rng.uniform(500.1, 500.4, m) if i in clumped
+                   else rng.uniform(0.0, 1000.0, m))
+        q[:, 1] = rng.uniform(cy * cs + 0.5, (cy + 1) * cs - 0.5, m)
+        q[:, 2] = rng.uniform(cz * cs + 0.5, (cz + 1) * cs - 0.5, m)
+        parts.append(q)
+        hq = np.empty((40, 3), np.float32)
+        hq[:, 0] = rng.uniform(490.0, 510.0, 40) if i in clumped else rng.uniform(0, 1000, 40)
+        hq[:, 1] = rng.uniform(cy * cs, (cy + 1) * cs, 40)
+        hq[:, 2] = rng.uniform(cz * cs, (cz + 1) * cs, 40)
+        halos.append(hq)
+    p = np.concatenate(parts)
+    p = p[rng.permutation(p.shape[0])]
+    got_len = np.bincount(row_of(p), minlength=G * G)[rows]
+    assert got_len.tolist() == list(lengths)
+    return p, np.concatenate(halos)
+
+
+@pytest.mark.parametrize("x_split", [16, 3, 1])
+def test_long_rows(eng, x_split):
+    """Rows of the last sort pass just under, at and over the LDS row size
+    (4096 particles) and over several 8192-particle tiles, along x uniform or
+    all in one x cell, with 1024, 192 and 64 cells along x: rows over 4096
+    particles take the tiled long-row path."""
+    lengths = [4095, 4096, 4097, 8192, 8193, 3 * 8192 + 5, 30_001, 12_000]
+    p, h = long_rows_input(31, lengths, clumped={7})
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 8)
+    want, _ = oracle_chunks([p], h, r2)
+    eng.set_option("cell_size_milli", 15625)
+    eng.set_option("x_split", x_split)
+    try:
+        got = run_engine(eng, [p], h, r2)
+    finally:
+        eng.set_option("cell_size_milli", 0)
+        eng.set_option("x_split", 0)
+    assert_counts_equal(got, want, h)
+    assert want.sum() > 0
+
+
 def test_chunk_accumulation_and_reset(eng):
     """Counts add over chunks (empty and one-particle chunks included) and
     restart from zero after reset."""
