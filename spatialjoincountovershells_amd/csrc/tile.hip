@@ -38,7 +38,7 @@ constexpr int kHaloWords = 12;  // sx sy sz safe lx hx ly hy lz hz mult (padded)
 constexpr int kWaveWords = 64 * kHaloWords + 128 + 64 + 64 + kTileBinWords;
 constexpr uint64_t kItemPairs = 1ull << 22;  // pair tests a work item aims at
 constexpr uint32_t kMaxParts = 256;
-constexpr uint32_t kTileGrid = 2048;         // persistent wavefronts
+constexpr uint32_t kTileGrid = 3072;         // persistent wavefronts
 constexpr int kHaloBatch = 8;   // halos whose LDS reads go out together
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -117,15 +117,16 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
   extern __shared__ __align__(16) uint32_t smem[];
   const int K = sh.nrads;
   const int kpad = (K + 3) & ~3;
-  // LDS: [lut 2*kLutSize][r2 kpad][halos 64*kHaloWords][rows 64*2][valid-halo
-  // list 64][representatives 64][bins kTileBinWords]
-  ShellEntry* s_lut = reinterpret_cast<ShellEntry*>(smem);
-  float* s_r2 = reinterpret_cast<float*>(smem + 2 * kLutSize);
+  // LDS: [r2 kpad][halos 64*kHaloWords][rows 64*2][valid-halo list 64]
+  // [representatives 64][bins kTileBinWords].  The shell table stays in
+  // global memory (L1-resident): a copy per wavefront would cost 8 KB of LDS
+  // and about half of the wavefronts a CU can hold.
+  const ShellEntry* __restrict__ g_lut = sh.lut;
+  float* s_r2 = reinterpret_cast<float*>(smem);
   for (int i = threadIdx.x; i < K; i += 64) s_r2[i] = sh.r2[i];
-  for (int i = threadIdx.x; i < sh.nlut; i += 64) s_lut[i] = sh.lut[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  uint32_t* wbase = smem + 2 * kLutSize + kpad;
+  uint32_t* wbase = smem + kpad;
   float* hd = reinterpret_cast<float*>(wbase);
   uint32_t* rows = wbase + 64 * kHaloWords;  // [64] first particle, [64] prefix
   uint32_t* hl = rows + 128;                 // the image's valid halos
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
 #pragma unroll
               for (int u = 0; u < kHaloBatch; ++u) {
                 const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
-                e[u] = s_lut[hit[u] ? max(b, 0) : 0];
+                e[u] = g_lut[hit[u] ? max(b, 0) : 0];
               }
 #pragma unroll
               for (int u = 0; u < kHaloBatch; ++u) {
@@ -422,7 +423,7 @@ hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh
   if (e == hipSuccess) e = fill_u32(queue, 0u, 1, s);
   if (e != hipSuccess) return e;
   const size_t kpad = (size_t)((sh.nrads + 3) & ~3);
-  const size_t lds = 4 * (2 * (size_t)kLutSize + kpad + (size_t)kWaveWords);
+  const size_t lds = 4 * (kpad + (size_t)kWaveWords);
   const int acc = accumulate ? 1 : 0;
   if (sh.one_step) {
     if (measure)
