@@ -303,10 +303,13 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
 #pragma unroll
             for (int u = 0; u < SEG; ++u) {
               dist[u] = dpps_distance(px[u], py[u], pz[u], sx, sy, sz);
+              // a pair outside the box gets distance +inf: the branch then
+              // writes only a float, and the hit flags below stay lane masks
+              if (dist[u] < sh.r2max && !(dist[u] < safe) &&
+                  !(!(px[u] < lx) && !(hx < px[u]) && !(py[u] < ly) && !(hy < py[u]) &&
+                    !(pz[u] < lz) && !(hz < pz[u])))
+                dist[u] = INFINITY;
               hit[u] = ((uint32_t)u < slen) & (dist[u] < sh.r2max);
-              if (hit[u] && !(dist[u] < safe))
-                hit[u] = !(px[u] < lx) && !(hx < px[u]) && !(py[u] < ly) && !(hy < py[u]) &&
-                         !(pz[u] < lz) && !(hz < pz[u]);
             }
             ShellEntry e[SEG];
 #pragma unroll
