@@ -1,14 +1,17 @@
-// primitives.hip -- device-wide scans, the LSD radix sort that replaces the
+// primitives.hip -- device-wide scans, the cell sort that replaces the
 // reference's STR bulk load, and the cell table.
 //
 // The reference builds its index with a parallel LSD radix sort over FP32
 // coordinate keys, one axis after another, then packs runs of 10 into tree
 // nodes (src/objects/ArraySTR/Build.c:845-963).  Here the index is a cell
-// grid: one stable LSD radix sort of (cell key, x, y, z) by 8-bit digits,
-// then a table of where every cell ends.  Each sort pass is three kernels:
-// per-tile digit histograms, one exclusive scan over them, and a scatter
-// that ranks each tile in LDS (wave ballots give the stable rank) and writes
-// every digit's run out contiguously.
+// grid: the particles in cell order plus a table of where every cell ends.
+// Grids of up to 256 x 256 x 1024 cells use a three-pass MSD sort whose
+// digits are single coordinates' cells (cz over the chunk, cy inside each
+// cz slab, cx inside each (cz, cy) row); each pass counts per tile, scans,
+// and scatters each tile through LDS so every digit's run is written out
+// contiguously.  Finer grids use one stable LSD radix sort of (cell key, x,
+// y, z) by 8-bit digits (wave ballots give the stable rank).
+
 #include "primitives.h"
 
 namespace sjcs {
