@@ -1547,6 +1547,10 @@ struct ServerSlot {
   DevBuf xyz, key[2], x[2], y[2], z[2], row[2], ws, r2, lut, c32, c64;
   void* pin = nullptr;
   size_t pin_bytes = 0;
+  // the finished counts are being copied out of `pin` (pieces left; under
+  // the server mutex): the slot stays taken until the last piece is done
+  bool copying = false;
+  int pieces_left = 0;
   std::shared_ptr<ServerRequest> req;
   void release() {
     xyz.release();
@@ -1588,6 +1592,19 @@ struct sjcs_server {
   bool stop = false;
   std::thread worker;
   std::string err;
+  // copy-out pool: a request's counts go from its slot's page-locked buffer
+  // to the caller's memory in pieces on these threads, so the worker keeps
+  // launching while first-touch page faults of the callers' buffers are
+  // taken on several cores at once
+  struct CopyJob {
+    ServerSlot* slot;
+    size_t off, bytes;
+  };
+  std::mutex cmu;
+  std::condition_variable ccv;
+  std::deque<CopyJob> cjobs;
+  bool cstop = false;
+  std::vector<std::thread> copiers;
 };
 
 namespace {
@@ -1672,18 +1689,81 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
 }
 #undef SRVCALL
 
+// Bytes of counts one copy-out job moves, and the copy-out threads.
+constexpr size_t kCopyPiece = (size_t)4 << 20;
+constexpr int kCopyThreads = 4;
+
+// The request on slot sl is over (status set): frees the slot and wakes the
+// waiters.  Caller holds sv->mu.
+void finish_request(sjcs_server* sv, ServerSlot& sl) {
+  ServerRequest& r = *sl.req;
+  r.t_done = now_s(sv->t0);
+  r.xyz.clear();
+  r.xyz.shrink_to_fit();
+  sv->used -= r.need;
+  sl.req.reset();
+  sl.copying = false;
+  sv->cv.notify_all();
+}
+
+// Copy-out thread: pieces of finished requests' counts, page-locked buffer
+// -> caller's memory; the last piece of a request completes it.
+void copy_loop(sjcs_server* sv) {
+  for (;;) {
+    sjcs_server::CopyJob job;
+    {
+      std::unique_lock<std::mutex> lk(sv->cmu);
+      sv->ccv.wait(lk, [&] { return sv->cstop || !sv->cjobs.empty(); });
+      if (sv->cjobs.empty()) return;  // stopping, nothing left
+      job = sv->cjobs.front();
+      sv->cjobs.pop_front();
+    }
+    ServerSlot& sl = *job.slot;
+    char* dst = reinterpret_cast<char*>(sl.req->out);
+    memcpy(dst + job.off, static_cast<const char*>(sl.pin) + job.off, job.bytes);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (--sl.pieces_left == 0) {
+      sl.req->status = 0;
+      finish_request(sv, sl);
+    }
+  }
+}
+
 void serve_loop(sjcs_server* sv) {
   (void)hipSetDevice(sv->device);
   for (;;) {
     bool busy = false;
-    // retire finished requests
+    // retire finished requests: their counts go to the copy-out pool
     for (ServerSlot& sl : sv->slots) {
-      if (!sl.req) continue;
-      busy = true;
+      std::shared_ptr<ServerRequest> rq;
+      {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (!sl.req) continue;
+        busy = true;
+        if (sl.copying) continue;
+        rq = sl.req;
+      }
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
-      ServerRequest& r = *sl.req;
-      if (q == hipSuccess) memcpy(r.out, sl.pin, 8 * (size_t)r.n * (size_t)r.nrads);
+      ServerRequest& r = *rq;
+      const size_t bytes = 8 * (size_t)r.n * (size_t)r.nrads;
+      if (q == hipSuccess && bytes > 0) {
+        const int pieces = (int)((bytes + kCopyPiece - 1) / kCopyPiece);
+        {
+          std::lock_guard<std::mutex> lk(sv->mu);
+          sl.copying = true;
+          sl.pieces_left = pieces;
+        }
+        {
+          std::lock_guard<std::mutex> lk(sv->cmu);
+          for (int i = 0; i < pieces; ++i) {
+            const size_t off = (size_t)i * kCopyPiece;
+            sv->cjobs.push_back({&sl, off, std::min(kCopyPiece, bytes - off)});
+          }
+        }
+        sv->ccv.notify_all();
+        continue;
+      }
       std::lock_guard<std::mutex> lk(sv->mu);
       if (q == hipSuccess) {
         r.status = 0;
@@ -1691,12 +1771,7 @@ void serve_loop(sjcs_server* sv) {
         r.err = std::string("request: ") + hipGetErrorString(q);
         r.status = SJCS_ERR_DEVICE;
       }
-      r.t_done = now_s(sv->t0);
-      r.xyz.clear();
-      r.xyz.shrink_to_fit();
-      sv->used -= r.need;
-      sl.req.reset();
-      sv->cv.notify_all();
+      finish_request(sv, sl);
     }
     // admit queued requests in order while a slot and the budget allow
     for (;;) {
@@ -1729,6 +1804,7 @@ void serve_loop(sjcs_server* sv) {
         sv->used -= next->need;
         sv->cv.notify_all();
       } else {
+        std::lock_guard<std::mutex> lk(sv->mu);
         free_slot->req = next;
       }
     }
@@ -1775,6 +1851,7 @@ int sjcs_server_create(sjcs_index* ix, int slots, uint64_t mem_budget, sjcs_serv
     return hip_fail(e, h, "server streams");
   }
   sv->worker = std::thread(serve_loop, sv);
+  for (int i = 0; i < kCopyThreads; ++i) sv->copiers.emplace_back(copy_loop, sv);
   *out = sv;
   return SJCS_OK;
 }
@@ -1865,6 +1942,13 @@ void sjcs_server_destroy(sjcs_server* sv) {
     sv->cv.notify_all();
   }
   if (sv->worker.joinable()) sv->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(sv->cmu);
+    sv->cstop = true;
+  }
+  sv->ccv.notify_all();
+  for (std::thread& t : sv->copiers)
+    if (t.joinable()) t.join();
   (void)hipSetDevice(sv->device);
   for (ServerSlot& sl : sv->slots) {
     if (sl.st) (void)hipStreamSynchronize(sl.st);
