@@ -59,6 +59,12 @@ hipError_t ensure(DevBuf& b, size_t bytes) {
   return hipSuccess;
 }
 
+// Grow-only device buffer that grows by half at least.
+hipError_t ensure_grow(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && bytes) return hipSuccess;
+  return ensure(b, std::max(bytes, b.bytes + b.bytes / 2));
+}
+
 struct ShellState {
   int nrads = 0;
   std::vector<float> r2;
@@ -1544,7 +1550,7 @@ constexpr uint64_t kServeSortMin = 4096;
 struct ServerSlot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
-  DevBuf xyz, key[2], x[2], y[2], z[2], row[2], ws, r2, lut, c32, c64;
+  DevBuf xyz, key[2], x[2], y[2], z[2], row[2], ws, c32, c64;  // xyz: the request's upload
   void* pin = nullptr;
   size_t pin_bytes = 0;
   // the finished counts are being copied out of `pin` (pieces left; under
@@ -1562,8 +1568,6 @@ struct ServerSlot {
       row[i].release();
     }
     ws.release();
-    r2.release();
-    lut.release();
     c32.release();
     c64.release();
     if (pin) (void)hipHostFree(pin);
@@ -1625,36 +1629,38 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   const int K = r.nrads;
   const size_t b = 4 * (size_t)(n ? n : 1);
   const size_t nc = (size_t)n * (size_t)K;
-  SRVCALL(ensure(sl.xyz, 3 * b), "hipMalloc");
+  // one upload from page-locked memory: [halos 12n | thresholds 4K | shell
+  // table], each part 256-byte aligned
+  const size_t off_r2 = (12 * (size_t)n + 255) & ~(size_t)255;
+  const size_t off_lut = (off_r2 + 4 * (size_t)K + 255) & ~(size_t)255;
+  const size_t in_bytes = off_lut + sizeof(ShellEntry) * r.lut.size();
+  // slot buffers grow by half at least, so requests of other shapes seldom
+  // reallocate (a device free synchronises the whole device)
+  SRVCALL(ensure_grow(sl.xyz, in_bytes), "hipMalloc");
   for (int i = 0; i < 2; ++i) {
-    SRVCALL(ensure(sl.key[i], b), "hipMalloc");
-    SRVCALL(ensure(sl.x[i], b), "hipMalloc");
-    SRVCALL(ensure(sl.y[i], b), "hipMalloc");
-    SRVCALL(ensure(sl.z[i], b), "hipMalloc");
-    SRVCALL(ensure(sl.row[i], b), "hipMalloc");
+    SRVCALL(ensure_grow(sl.key[i], b), "hipMalloc");
+    SRVCALL(ensure_grow(sl.x[i], b), "hipMalloc");
+    SRVCALL(ensure_grow(sl.y[i], b), "hipMalloc");
+    SRVCALL(ensure_grow(sl.z[i], b), "hipMalloc");
+    SRVCALL(ensure_grow(sl.row[i], b), "hipMalloc");
   }
-  SRVCALL(ensure(sl.ws, 4 * radix_workspace_words(n)), "hipMalloc");
-  SRVCALL(ensure(sl.r2, 4 * (size_t)K), "hipMalloc");
-  SRVCALL(ensure(sl.lut, sizeof(ShellEntry) * r.lut.size()), "hipMalloc");
-  SRVCALL(ensure(sl.c32, 4 * (nc ? nc : 1)), "hipMalloc");
-  SRVCALL(ensure(sl.c64, 8 * (nc ? nc : 1)), "hipMalloc");
-  const size_t pin = std::max(12 * (size_t)n, 8 * nc) + 64;
+  SRVCALL(ensure_grow(sl.ws, 4 * radix_workspace_words(n)), "hipMalloc");
+  SRVCALL(ensure_grow(sl.c32, 4 * (nc ? nc : 1)), "hipMalloc");
+  SRVCALL(ensure_grow(sl.c64, 8 * (nc ? nc : 1)), "hipMalloc");
+  const size_t pin = std::max(in_bytes, 8 * nc) + 64;
   if (sl.pin_bytes < pin) {
+    const size_t grown = std::max(pin, sl.pin_bytes + sl.pin_bytes / 2);
     if (sl.pin) (void)hipHostFree(sl.pin);
     sl.pin = nullptr;
     sl.pin_bytes = 0;
-    SRVCALL(hipHostMalloc(&sl.pin, pin, hipHostMallocDefault), "hipHostMalloc");
-    sl.pin_bytes = pin;
+    SRVCALL(hipHostMalloc(&sl.pin, grown, hipHostMallocDefault), "hipHostMalloc");
+    sl.pin_bytes = grown;
   }
-  if (n) memcpy(sl.pin, r.xyz.data(), 12 * (size_t)n);
-  if (n)
-    SRVCALL(hipMemcpyAsync(sl.xyz.p, sl.pin, 12 * (size_t)n, hipMemcpyHostToDevice, sl.st),
-            "hipMemcpy");
-  SRVCALL(hipMemcpyAsync(sl.r2.p, r.r2.data(), 4 * (size_t)K, hipMemcpyHostToDevice, sl.st),
-          "hipMemcpy");
-  SRVCALL(hipMemcpyAsync(sl.lut.p, r.lut.data(), sizeof(ShellEntry) * r.lut.size(),
-                         hipMemcpyHostToDevice, sl.st),
-          "hipMemcpy");
+  char* pin_in = static_cast<char*>(sl.pin);
+  if (n) memcpy(pin_in, r.xyz.data(), 12 * (size_t)n);
+  memcpy(pin_in + off_r2, r.r2.data(), 4 * (size_t)K);
+  memcpy(pin_in + off_lut, r.lut.data(), sizeof(ShellEntry) * r.lut.size());
+  SRVCALL(hipMemcpyAsync(sl.xyz.p, sl.pin, in_bytes, hipMemcpyHostToDevice, sl.st), "hipMemcpy");
   SRVCALL(halo_keys(sl.xyz.as<float>(), 12, n, sl.key[0].as<uint32_t>(), sl.x[0].as<float>(),
                     sl.y[0].as<float>(), sl.z[0].as<float>(), sl.row[0].as<uint32_t>(), sl.st),
           "halo keys");
@@ -1673,8 +1679,8 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   hv.z = sl.z[sel].as<float>();
   hv.orig = sl.row[sel].as<uint32_t>();
   hv.n = n;
-  r.shells.r2 = sl.r2.as<float>();
-  r.shells.lut = sl.lut.as<ShellEntry>();
+  r.shells.r2 = reinterpret_cast<float*>(sl.xyz.as<char>() + off_r2);
+  r.shells.lut = reinterpret_cast<ShellEntry*>(sl.xyz.as<char>() + off_lut);
   const IndexView ix = sv->ix->ib.view();
   const double side = 2.0 * (double)r.shells.rbox / 1000.0;
   const int lanes_auto = !((double)ix.n * side * side * side <= 4096.0) ? 64 : 16;
