@@ -1546,6 +1546,10 @@ size_t request_bytes(uint64_t n, int nrads) {
 // Below this many halos a request is counted in input order (a sort costs
 // more launches than it saves).
 constexpr uint64_t kServeSortMin = 4096;
+// A request's halos are ordered by the top 15 bits of their Morton keys
+// (cells ~31 units wide): two sort passes instead of four, and the count's
+// neighbouring halos still share most cell rows.
+constexpr int kServeKeyBits = 15;
 
 struct ServerSlot {
   hipStream_t st = nullptr;
@@ -1662,7 +1666,8 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   memcpy(pin_in + off_lut, r.lut.data(), sizeof(ShellEntry) * r.lut.size());
   SRVCALL(hipMemcpyAsync(sl.xyz.p, sl.pin, in_bytes, hipMemcpyHostToDevice, sl.st), "hipMemcpy");
   SRVCALL(halo_keys(sl.xyz.as<float>(), 12, n, sl.key[0].as<uint32_t>(), sl.x[0].as<float>(),
-                    sl.y[0].as<float>(), sl.z[0].as<float>(), sl.row[0].as<uint32_t>(), sl.st),
+                    sl.y[0].as<float>(), sl.z[0].as<float>(), sl.row[0].as<uint32_t>(), sl.st,
+                    kServeKeyBits),
           "halo keys");
   int sel = 0;
   if (n >= kServeSortMin) {
@@ -1671,7 +1676,8 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
                             sl.row[0].as<uint32_t>()},
                            {sl.x[1].as<uint32_t>(), sl.y[1].as<uint32_t>(), sl.z[1].as<uint32_t>(),
                             sl.row[1].as<uint32_t>()}};
-    SRVCALL(radix_sort_pairs(keys, pay, 4, n, 30, sl.ws.as<uint32_t>(), sl.st, &sel), "halo sort");
+    SRVCALL(radix_sort_pairs(keys, pay, 4, n, kServeKeyBits, sl.ws.as<uint32_t>(), sl.st, &sel),
+            "halo sort");
   }
   HaloView hv;
   hv.x = sl.x[sel].as<float>();
@@ -1697,7 +1703,7 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
 
 // Bytes of counts one copy-out job moves, and the copy-out threads.
 constexpr size_t kCopyPiece = (size_t)4 << 20;
-constexpr int kCopyThreads = 4;
+constexpr int kCopyThreads = 8;
 
 // The request on slot sl is over (status set): frees the slot and wakes the
 // waiters.  Caller holds sv->mu.

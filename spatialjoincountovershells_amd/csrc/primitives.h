@@ -109,9 +109,10 @@ hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key,
                          float* x, float* y, float* z, hipStream_t s);
 
 // Halo centres (3 FP32 every stride_bytes) -> Morton keys (10 bits per
-// axis) + SoA + row.
+// axis; the top key_bits of the 30, for a coarser order in fewer sort
+// passes) + SoA + row.
 hipError_t halo_keys(const float* pts, size_t stride_bytes, uint64_t n, uint32_t* key, float* x,
-                     float* y, float* z, uint32_t* row, hipStream_t s);
+                     float* y, float* z, uint32_t* row, hipStream_t s, int key_bits = 30);
 
 // *flag = 1 when the sorted keys hold a run of at least `run` equal keys,
 // else 0.

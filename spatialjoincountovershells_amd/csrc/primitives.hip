@@ -818,7 +818,7 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 }
 
 __global__ void halo_keys_kernel(const char* __restrict__ pts, uint64_t stride, uint64_t n,
-                                 uint32_t* __restrict__ key, float* __restrict__ x,
+                                 int shift, uint32_t* __restrict__ key, float* __restrict__ x,
                                  float* __restrict__ y, float* __restrict__ z,
                                  uint32_t* __restrict__ row) {
   const float inv = 1024.0f / 1000.0f;
@@ -829,7 +829,7 @@ __global__ void halo_keys_kernel(const char* __restrict__ pts, uint64_t stride, 
     uint32_t cx = (uint32_t)cell_of(hx, inv, 1023);
     uint32_t cy = (uint32_t)cell_of(hy, inv, 1023);
     uint32_t cz = (uint32_t)cell_of(hz, inv, 1023);
-    key[i] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    key[i] = (spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2)) >> shift;
     x[i] = hx;
     y[i] = hy;
     z[i] = hz;
@@ -973,10 +973,12 @@ hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key, fl
 }
 
 hipError_t halo_keys(const float* pts, size_t stride_bytes, uint64_t n, uint32_t* key, float* x,
-                     float* y, float* z, uint32_t* row, hipStream_t s) {
+                     float* y, float* z, uint32_t* row, hipStream_t s, int key_bits) {
   if (n == 0) return hipSuccess;
+  if (key_bits < 1 || key_bits > 30) return hipErrorInvalidValue;
   halo_keys_kernel<<<grid_blocks(n), 256, 0, s>>>(reinterpret_cast<const char*>(pts),
-                                                  (uint64_t)stride_bytes, n, key, x, y, z, row);
+                                                  (uint64_t)stride_bytes, n, 30 - key_bits, key,
+                                                  x, y, z, row);
   return hipGetLastError();
 }
 
