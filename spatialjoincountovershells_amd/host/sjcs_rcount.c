@@ -12,7 +12,8 @@
  *  - the device is set up (runtime start, engine, pinned staging) on a
  *    thread of its own while the halo file is parsed;
  *  - a particle file goes from disk to the device in 64 MiB pieces, each
- *    read by parallel preads into page-locked memory and copied from there;
+ *    read by parallel preads into page-locked memory and copied from there
+ *    while the next piece is read;
  *  - the counts come back in batches of rows that the result writer asks
  *    for while its workers format the previous batch.
  * The per-file "particles reading time" line reports disk to device memory,
@@ -108,22 +109,59 @@ struct chunk_buf {
     long read_ms;
 };
 
+typedef struct {
+    sjcs_engine *e;
+    char *dst;
+    const void *src;
+    size_t len;
+    int rc;
+} piece_copy;
+
+static void *piece_copy_run(void *arg)
+{
+    piece_copy *c = arg;
+    c->rc = sjcs_copy_to_device(c->e, c->dst, c->src, c->len);
+    return NULL;
+}
+
 /* Bytes [off, off + bytes) of fname into device memory at d, through the
- * two pinned pieces.  0, the sjcs_pread_par status, or -3 (copy failed). */
+ * two pinned pieces: piece i is copied to the device on a helper thread
+ * while piece i + 1 is read into the other one.  0, the sjcs_pread_par
+ * status, or -3 (copy failed). */
 static int file_to_device(sjcs_engine *e, void *pin[2], const char *fname, uint64_t off,
                           uint64_t bytes, char *d, int nthreads)
 {
     uint64_t done = 0;
-    int k = 0;
+    int k = 0, copying = 0, rc = 0;
+    pthread_t th;
+    piece_copy job;
     while (done < bytes) {
         uint64_t len = bytes - done < PIECE_BYTES ? bytes - done : PIECE_BYTES;
-        int rc = sjcs_pread_par(fname, off + done, len, pin[k], nthreads);
+        rc = sjcs_pread_par(fname, off + done, len, pin[k], nthreads);
+        if (copying) {  /* the other piece's copy; pin[k ^ 1] is free after it */
+            pthread_join(th, NULL);
+            copying = 0;
+            if (job.rc != SJCS_OK)
+                return -3;
+        }
         if (rc != 0)
             return rc;
-        if (sjcs_copy_to_device(e, d + done, pin[k], (size_t)len) != SJCS_OK)
+        job.e = e;
+        job.dst = d + done;
+        job.src = pin[k];
+        job.len = (size_t)len;
+        job.rc = SJCS_OK;
+        if (pthread_create(&th, NULL, piece_copy_run, &job) == 0)
+            copying = 1;
+        else if (sjcs_copy_to_device(e, d + done, pin[k], (size_t)len) != SJCS_OK)
             return -3;
         done += len;
         k ^= 1;
+    }
+    if (copying) {
+        pthread_join(th, NULL);
+        if (job.rc != SJCS_OK)
+            return -3;
     }
     return 0;
 }
