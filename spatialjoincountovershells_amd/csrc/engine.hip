@@ -1511,10 +1511,12 @@ void sjcs_index_destroy(sjcs_index* ix) {
 // worker thread admits queued requests in arrival order while a slot (own
 // stream and buffers) is free and the requests in flight fit the device
 // memory budget; admitted requests run concurrently on their streams.  The
-// worker retires finished requests (event query), copies their counts to the
-// caller's buffer and admits the next ones, so requests that arrive while
-// others run start as soon as room frees up.  Each request is computed on its
-// own, so its counts do not depend on what else is in flight.
+// worker retires finished requests (event query), hands their counts to a
+// pool of copy-out threads (page-locked slot buffer -> caller's buffer, in
+// pieces) and admits the next ones, so requests that arrive while others run
+// start as soon as room frees up; a slot is free again once its counts are
+// copied out.  Each request is computed on its own, so its counts do not
+// depend on what else is in flight.
 
 namespace {
 
