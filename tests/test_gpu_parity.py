@@ -572,6 +572,57 @@ def test_server_alone_equals_mix(built, eng):
         ix.close()
 
 
+def test_server_concurrent_submitters(built, eng):
+    """Four host threads submit and await requests on one server at the same
+    time (counts copied out in pieces by the server's copy threads while
+    others are launched): every result equals the same request served
+    alone."""
+    import threading
+    rng = np.random.default_rng(29)
+    p = rng.random((200_000, 3), dtype=np.float32) * np.float32(1000.0)
+    ix = built.ArraySTR(eng, p.shape[0])
+    ix.insert(p)
+    ix.make_tree()
+    reqs = []
+    for i in range(48):
+        n = int(10 ** rng.uniform(0.0, 4.7))
+        rmax = float(rng.uniform(1.0, 10.0))
+        k = int(rng.integers(8, 65))
+        _, r2 = t.oracle_thresholds(rmax / 5000.0, rmax, k)
+        reqs.append((rng.random((n, 3), dtype=np.float32) * np.float32(1000.0), r2))
+    sv = built.Server(ix, slots=4)
+    try:
+        alone = []
+        for h, r2 in reqs:
+            tk = sv.submit(h, r2)
+            alone.append(sv.wait(tk).copy())
+            sv.release(tk)
+        got = [None] * len(reqs)
+        errors = []
+
+        def worker(w):
+            try:
+                mine = list(range(w, len(reqs), 4))
+                tks = [(i, sv.submit(*reqs[i])) for i in mine]
+                for i, tk in tks:
+                    got[i] = sv.wait(tk).copy()
+                    sv.release(tk)
+            except Exception as exc:  # reported below, on the main thread
+                errors.append(exc)
+
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        assert not errors, errors
+        for i, a in enumerate(alone):
+            assert np.array_equal(got[i], a), i
+    finally:
+        sv.close()
+        ix.close()
+
+
 def test_degenerate_sizes(eng):
     _, r2 = t.oracle_thresholds(0.1, 5.0, 4)
     p, h = gen_uniform(17, 5_000, 50)
