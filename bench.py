@@ -14,7 +14,7 @@ chunk loop.
 
 Config c2z is the same search with the particles given as a compressed
 (Gadget) snapshot: a step then starts from the device-resident uint16
-triplets and decodes them on the device before the sort.
+triplets, which the cell sort's first pass decodes as it reads them.
 
 ``value`` = W / (seconds per step), W = exact number of in-box
 (halo image, particle) pair tests of the chunk -- the reference's
@@ -436,7 +436,6 @@ def main():
     ms_step = elapsed * 1000.0 / args.steps
 
     counted = int(eng.counts().sum())
-    upload_ms = float(phase[:, 0].mean())
     build_ms = float(phase[:, 1].mean())
     count_ms = float(phase[:, 2].mean())
     log(f"[bench] W={w} counted={counted} step={ms_step:.3f} ms build={build_ms:.3f} "
@@ -449,8 +448,9 @@ def main():
     count_bytes = 12 * n + 12 * nh + 4 * k * nh * (2 if cfg["chunks"] > 1 else 1)
     count_gbs = count_bytes / (count_ms * 1e-3) / 1e9
     count_tflops = 8 * w / (count_ms * 1e-3) / 1e12
-    # index build: packed xyz read once, sorted SoA xyz written once
-    build_bytes = 24 * n
+    # index build: packed xyz (12 B) or compressed triplets (6 B) read once,
+    # sorted SoA xyz written once
+    build_bytes = (6 if cfg.get("compressed") else 12) * n + 12 * n
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     dominant = "count" if count_ms >= build_ms else "index_build"
     traffic = None
@@ -469,20 +469,6 @@ def main():
                 "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": "cell sort (cs_* kernels)",
                 "bytes_per_launch": build_bytes}
-
-    decode = None
-    if cfg.get("compressed"):
-        # device decode: 6 B of triplets read, 12 B of FP32 xyz written per particle
-        decode_bytes = 18 * n
-        dec_gbs = decode_bytes / (upload_ms * 1e-3) / 1e9
-        decode = {"kernel": "gadget_decode_kernel", "ms": round(upload_ms, 4),
-                  "bytes_per_launch": decode_bytes, "achieved": round(dec_gbs, 1),
-                  "unit": "GB/s", "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
-                  "traffic": None}
-        tfile = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tfile):
-            with open(tfile) as f:
-                decode["traffic"] = json.load(f).get(args.config, {}).get("decode")
 
     # measured VALU issue of the count kernel (rocprofv3 counters, profiles/)
     vfile = os.path.join(ROOT, "profiles", "valu.json")
@@ -524,7 +510,6 @@ def main():
                    "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
                    "pairs_counted": counted, "parallelism": "single GPU"},
         "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4),
-                      **({"decode": round(upload_ms, 4)} if cfg.get("compressed") else {}),
                       "of": ("the last chunk of a step" if cfg["chunks"] > 1 else "the chunk") +
                             (" (graph replay: sort and count timed together as count)"
                              if args.graph or cfg.get("graph") else "")},
@@ -536,7 +521,6 @@ def main():
                          "valu_lane_slots_per_pair_test":
                              valu and valu.get("valu_lane_slots_per_pair_test")},
         "autotune": tuned,
-        "decode_kernel": decode,
         "wall": wall,
         "roofline": roof,
         "cpu_baseline": cpu,

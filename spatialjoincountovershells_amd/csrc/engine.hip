@@ -65,6 +65,12 @@ hipError_t ensure_grow(DevBuf& b, size_t bytes) {
   return ensure(b, std::max(bytes, b.bytes + b.bytes / 2));
 }
 
+ParticleSource packed(const float* xyz) {
+  ParticleSource s;
+  s.xyz = xyz;
+  return s;
+}
+
 struct ShellState {
   int nrads = 0;
   std::vector<float> r2;
@@ -247,7 +253,8 @@ struct sjcs_engine {
   // or a value captured in a graph may have changed.
   struct ChunkGraph {
     hipGraphExec_t exec = nullptr;
-    const float* in = nullptr;
+    const void* in = nullptr;     // the piece's particles (packed xyz or triplets)
+    GadgetDecode gd{};            // triplets: the decode's constants
     uint64_t n = 0;
     uint64_t epoch = 0;
   };
@@ -568,22 +575,26 @@ int reserve_index(sjcs_engine* e, IndexBuffers& ib, uint64_t n, Grid g, bool* gr
   return SJCS_OK;
 }
 
-// Enqueues the index build of n particles at d_xyz into ib (sized by
-// reserve_index) on stream st; launches only.
-int enqueue_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
+// Enqueues the index build of the n particles of src into ib (sized by
+// reserve_index) on stream st; launches only.  Compressed triplets need the
+// per-axis sort (the packed-key sort reads decoded positions) and a file of
+// one tile per side.
+int enqueue_index(sjcs_engine* e, IndexBuffers& ib, const ParticleSource& src, uint64_t n, Grid g,
                   hipStream_t st) {
   const bool by_axis = sort_by_axis(e, g);
+  if (src.uv && (!by_axis || !src.gd.one_tile))
+    return fail(e, SJCS_ERR_STATE, "compressed particles reach the index build undecoded");
   if (by_axis) {
     float* xs[2] = {ib.x[0].as<float>(), ib.x[1].as<float>()};
     float* ys[2] = {ib.y[0].as<float>(), ib.y[1].as<float>()};
     float* zs[2] = {ib.z[0].as<float>(), ib.z[1].as<float>()};
-    HIPCALL(e, cell_sort_particles(d_xyz, n, g, xs, ys, zs, ib.cell_end.as<uint32_t>(),
+    HIPCALL(e, cell_sort_particles(src, n, g, xs, ys, zs, ib.cell_end.as<uint32_t>(),
                                    e->ws.as<uint32_t>(), st),
             "cell sort");
     ib.sel = 0;
     return SJCS_OK;
   }
-  HIPCALL(e, particle_keys(d_xyz, n, g, ib.key[0].as<uint32_t>(), ib.x[0].as<float>(),
+  HIPCALL(e, particle_keys(src.xyz, n, g, ib.key[0].as<uint32_t>(), ib.x[0].as<float>(),
                            ib.y[0].as<float>(), ib.z[0].as<float>(), st),
           "cell keys");
   uint32_t* keys[2] = {ib.key[0].as<uint32_t>(), ib.key[1].as<uint32_t>()};
@@ -604,12 +615,12 @@ int enqueue_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t
 // Builds the cell index of n particles at d_xyz (device, packed) into ib,
 // on stream st.  Buffers that must grow are reallocated only after all work
 // of the engine is done.
-int build_index(sjcs_engine* e, IndexBuffers& ib, const float* d_xyz, uint64_t n, Grid g,
+int build_index(sjcs_engine* e, IndexBuffers& ib, const ParticleSource& src, uint64_t n, Grid g,
                 hipStream_t st) {
   bool grew = false;
   int rc = reserve_index(e, ib, n, g, &grew);
   if (rc != SJCS_OK) return rc;
-  return enqueue_index(e, ib, d_xyz, n, g, st);
+  return enqueue_index(e, ib, src, n, g, st);
 }
 
 HaloView halo_view(const HaloState& h) {
@@ -740,7 +751,7 @@ int piece_done(sjcs_engine* e, uint64_t n);
 // captured on first use and replayed while nothing it holds has changed.
 // The sort waits for the slot's upload; it follows the previous count in
 // stream order.
-int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
+int graph_piece(sjcs_engine* e, const ParticleSource& src, uint64_t n, const Grid& g) {
   const int sl = e->slot;
   bool grew = false, use_tiles = false;
   int rc = reserve_index(e, e->ibs[sl], n, g, &grew);
@@ -751,13 +762,15 @@ int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
   HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
   sjcs_engine::ChunkGraph& cg = e->graphs[sl][e->counts_fresh ? 0 : 1];
-  if (!cg.exec || cg.in != d_xyz || cg.n != n || cg.epoch != e->epoch) {
+  const void* in = src.uv ? static_cast<const void*>(src.uv) : static_cast<const void*>(src.xyz);
+  if (!cg.exec || cg.in != in || cg.n != n || cg.epoch != e->epoch ||
+      (src.uv && memcmp(&cg.gd, &src.gd, sizeof(GadgetDecode)) != 0)) {
     if (cg.exec) (void)hipGraphExecDestroy(cg.exec);
     cg.exec = nullptr;
     hipGraph_t graph = nullptr;
     HIPCALL(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal),
             "hipStreamBeginCapture");
-    int rq = enqueue_index(e, e->ibs[sl], d_xyz, n, g, e->stream);
+    int rq = enqueue_index(e, e->ibs[sl], src, n, g, e->stream);
     hipError_t hc = hipSuccess;
     if (rq == SJCS_OK)
       hc = count_chunk(e, e->ibs[sl].view(), use_tiles, e->counts.as<uint32_t>(), !e->counts_fresh,
@@ -777,7 +790,8 @@ int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
       cg.exec = nullptr;
       return hip_fail(e, hi, "hipGraphInstantiate");
     }
-    cg.in = d_xyz;
+    cg.in = in;
+    cg.gd = src.uv ? src.gd : GadgetDecode{};
     cg.n = n;
     cg.epoch = e->epoch;
   }
@@ -789,7 +803,7 @@ int graph_piece(sjcs_engine* e, const float* d_xyz, uint64_t n, const Grid& g) {
 }
 
 // Sorts one piece into the current slot (bstream) and counts it (stream).
-int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+int add_piece(sjcs_engine* e, ParticleSource src, uint64_t n) {
   const int sl = e->slot;
   const uint64_t img = images_max(e);
   const uint64_t limit = e->fold_limit > 0 ? (uint64_t)e->fold_limit : 0xffffffffull;
@@ -798,7 +812,22 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
     if (rc != SJCS_OK) return rc;
   }
   const Grid g = choose_grid(n, e->shells.view.rbox, e->cell_size_milli, e->x_split);
-  if (e->graph) return graph_piece(e, d_xyz, n, g);
+  if (src.uv && (!sort_by_axis(e, g) || !src.gd.one_tile)) {
+    // the packed-key sort, and the cell sort for files of several tiles per
+    // side, need the positions first: decode into the slot's staging once
+    // the sort that last read it is done
+    const int sl = e->slot;
+    if (e->cin[sl].bytes < 12 * (size_t)n)
+      HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+    HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
+    HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+    HIPCALL(e, decode_gadget(src.uv, n, src.gd, e->cin[sl].as<float>(), e->bstream),
+            "decode compressed particles");
+    if (e->graph) HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
+    src = ParticleSource{};
+    src.xyz = e->cin[sl].as<float>();
+  }
+  if (e->graph) return graph_piece(e, src, n, g);
   bool use_tiles = false;
   int rt = ensure_tiles(e, &use_tiles);
   if (rt != SJCS_OK) return rt;
@@ -806,7 +835,7 @@ int add_piece(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   if (!e->overlap && e->last >= 0)
     HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[e->last], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[2], e->bstream), "hipEventRecord");
-  int rc = build_index(e, e->ibs[sl], d_xyz, n, g, e->bstream);
+  int rc = build_index(e, e->ibs[sl], src, n, g, e->bstream);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[3], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->built[sl], e->bstream), "hipEventRecord");
@@ -834,12 +863,19 @@ int piece_done(sjcs_engine* e, uint64_t n) {
 
 // A chunk, in pieces small enough for 32-bit particle offsets and for
 // per-piece counts below 2^32.
-int add_chunk(sjcs_engine* e, const float* d_xyz, uint64_t n) {
+int add_chunk(sjcs_engine* e, const ParticleSource& src, uint64_t n) {
   const uint64_t cap = std::min<uint64_t>(0x3fffffffull, 0xffffffffull / images_max(e));
   uint64_t done = 0;
   do {
     const uint64_t m = std::min(cap, n - done);
-    int rc = add_piece(e, d_xyz + 3 * done, m);
+    ParticleSource piece = src;
+    if (src.uv) {
+      piece.uv = src.uv + 3 * done;
+      piece.gd.id_start = src.gd.id_start + (int64_t)done;
+    } else {
+      piece.xyz = src.xyz + 3 * done;
+    }
+    int rc = add_piece(e, piece, m);
     if (rc != SJCS_OK) return rc;
     done += m;
   } while (done < n);
@@ -1022,7 +1058,7 @@ int sjcs_add_particles(sjcs_engine* e, const float* xyz, uint64_t n) {
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
-  return add_chunk(e, e->cin[sl].as<float>(), n);
+  return add_chunk(e, packed(e->cin[sl].as<float>()), n);
 }
 
 int sjcs_add_compressed_particles(sjcs_engine* e, const uint16_t* uv, uint64_t n,
@@ -1034,18 +1070,17 @@ int sjcs_add_compressed_particles(sjcs_engine* e, const uint16_t* uv, uint64_t n
   std::string err;
   if (!gadget_prepare(*p, n, &d, &err)) return fail(e, SJCS_ERR_INVALID, err);
   const int sl = e->slot;
-  // the slot's triplets are rewritten once its last decode (bstream, before
-  // `uploaded`) has read them
+  // the triplets go to the device as they are; the sort's first pass decodes
+  // them as it reads them (the slot's staging is rewritten only after the
+  // count that followed its last sort)
   rc = stage_chunk(e, uv, 6 * (size_t)n, e->cuv[sl]);
   if (rc != SJCS_OK) return rc;
-  if (e->cin[sl].bytes < 12 * (size_t)n)
-    HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
-  HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
-  HIPCALL(e, decode_gadget(e->cuv[sl].as<uint16_t>(), n, d, e->cin[sl].as<float>(), e->bstream),
-          "decode compressed particles");
   HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
-  return add_chunk(e, e->cin[sl].as<float>(), n);
+  ParticleSource src;
+  src.uv = e->cuv[sl].as<uint16_t>();
+  src.gd = d;
+  return add_chunk(e, src, n);
 }
 
 int sjcs_add_compressed_particles_device(sjcs_engine* e, const uint16_t* d_uv, uint64_t n,
@@ -1056,17 +1091,13 @@ int sjcs_add_compressed_particles_device(sjcs_engine* e, const uint16_t* d_uv, u
   GadgetDecode d;
   std::string err;
   if (!gadget_prepare(*p, n, &d, &err)) return fail(e, SJCS_ERR_INVALID, err);
-  const int sl = e->slot;
-  // the decode rewrites the slot's staging once the sort that read it is done
-  if (e->cin[sl].bytes < 12 * (size_t)n)
-    HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
-  HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
-  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+  // decoded inside the sort's first pass: no separate decode phase
   HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
-  HIPCALL(e, decode_gadget(d_uv, n, d, e->cin[sl].as<float>(), e->bstream),
-          "decode compressed particles");
   HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
-  return add_chunk(e, e->cin[sl].as<float>(), n);
+  ParticleSource src;
+  src.uv = d_uv;
+  src.gd = d;
+  return add_chunk(e, src, n);
 }
 
 int sjcs_decode_compressed_device(sjcs_engine* e, const uint16_t* d_uv, uint64_t n,
@@ -1090,7 +1121,7 @@ int sjcs_add_particles_device(sjcs_engine* e, const float* d_xyz, uint64_t n) {
   if (!d_xyz && n) return fail(e, SJCS_ERR_INVALID, "particles: null pointer");
   HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->ev[1], e->bstream), "hipEventRecord");
-  return add_chunk(e, d_xyz, n);
+  return add_chunk(e, packed(d_xyz), n);
 }
 
 int sjcs_sync(sjcs_engine* e) {
@@ -1208,7 +1239,7 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
     if (r != SJCS_OK) return r;
     for (int rep = 0; rep < 2; ++rep) {
       HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
-      r = build_index(e, e->ibs[0], d_xyz, n, g, e->stream);
+      r = build_index(e, e->ibs[0], packed(d_xyz), n, g, e->stream);
       if (r != SJCS_OK) return r;
       HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, scratch.as<uint32_t>(), false, nullptr,
                              e->stream),
@@ -1438,7 +1469,7 @@ int sjcs_index_build(sjcs_index* ix) {
   const Grid g = choose_grid(ix->n, 0.0f, e->cell_size_milli, e->x_split);
   rc = quiesce(e);  // the sort workspace is shared with the chunk path
   if (rc != SJCS_OK) return rc;
-  rc = build_index(e, ix->ib, ix->aos.as<float>(), ix->n, g, e->stream);
+  rc = build_index(e, ix->ib, packed(ix->aos.as<float>()), ix->n, g, e->stream);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
   ix->built = true;

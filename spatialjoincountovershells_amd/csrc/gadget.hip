@@ -16,70 +16,17 @@ namespace {
 constexpr int kDecodeThreads = 256;
 constexpr uint64_t kDecodeMaxBlocks = 1u << 16;
 
-// Integer division in double arithmetic.  Every id, quotient and remainder
-// here is an integer of magnitude below 2^50, so each is exact in a double:
-// q0 = trunc(a * inv), inv = 1/d rounded, is within 1/4 of a / d (two
-// roundings of 2^-53 each on a quotient below 2^50) and so off by at most
-// one; r = fma(-q0, d, a) is exact (q0*d < 2^51), and its sign fixes q0.
-// Cheaper on the vector ALU than 64-bit integer division or conversions.
-__device__ __forceinline__ void divmod_nonneg(double a, double d, double inv, double& q,
-                                              double& r) {
-  q = trunc(a * inv);
-  r = fma(-q, d, a);
-  if (r < 0.0) {
-    q -= 1.0;
-    r += d;
-  } else if (r >= d) {
-    q += 1.0;
-    r -= d;
-  }
-}
-
-// C's a / d and a % d: quotient truncated toward zero, remainder with the
-// sign of a.
-__device__ __forceinline__ void divmod_c(double a, double d, double inv, double& q, double& r) {
-  const bool neg = a < 0.0;
-  divmod_nonneg(fabs(a), d, inv, q, r);
-  if (neg) {
-    q = -q;
-    r = -r;
-  }
-}
-
-// kOneTile: one tile per side and 0 <= id - 1 < Ng3D for the whole file, so
-// the tile number is 0 and id - 1 is the number within the tile.
 template <bool kOneTile>
 __global__ __launch_bounds__(kDecodeThreads) void gadget_decode_kernel(
     const uint16_t* __restrict__ uv, uint64_t n, GadgetDecode g, float* __restrict__ xyz) {
   const uint64_t stride = (uint64_t)gridDim.x * kDecodeThreads;
-  const double ng1 = (double)g.ng1d, ng2 = (double)g.ng2d, ng3 = (double)g.ng3d;
-  const double base = (double)(g.id_start - 1);
   for (uint64_t i = (uint64_t)blockIdx.x * kDecodeThreads + threadIdx.x; i < n; i += stride) {
-    // grid point of id = id_start + i (GadId.c:70-86, 112-137)
-    const double m = base + (double)i;
-    double ga, gb, gc, ta = 0.0, tb = 0.0, tc = 0.0, r;
-    if (kOneTile) {
-      divmod_nonneg(m, ng2, g.inv_ng2d, ga, r);
-      divmod_nonneg(r, ng1, g.inv_ng1d, gb, gc);
-    } else {
-      double idt, idg;
-      divmod_c(m, ng3, g.inv_ng3d, idt, idg);
-      divmod_c(idg, ng2, g.inv_ng2d, ga, r);
-      divmod_c(r, ng1, g.inv_ng1d, gb, gc);
-      divmod_c(idt, (double)g.nt2d, g.inv_nt2d, ta, r);
-      divmod_c(r, (double)g.nt1d, g.inv_nt1d, tb, tc);
-    }
-    const int ip[3] = {(int)fma(ta, ng1, ga), (int)fma(tb, ng1, gb), (int)fma(tc, ng1, gc)};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float f = (float)ip[k];
-      // the library is built with -ffp-contract=off: the unfused form is
-      // two roundings, the fused one a single FMA
-      const float s = g.fused ? __builtin_fmaf(g.cell_interval, f, g.disp_min)
-                              : g.cell_interval * f + g.disp_min;
-      const double v = (double)s + g.unitsep * (double)uv[3 * i + k];
-      xyz[3 * i + k] = (float)v;
-    }
+    const uint32_t d[3] = {uv[3 * i], uv[3 * i + 1], uv[3 * i + 2]};
+    float p[3];
+    gadget_point<kOneTile>(g, i, d, p);
+    xyz[3 * i] = p[0];
+    xyz[3 * i + 1] = p[1];
+    xyz[3 * i + 2] = p[2];
   }
 }
 

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "common.h"
+#include "gadget.h"
 
 namespace sjcs {
 
@@ -96,13 +97,23 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay,
                             uint64_t n, int key_bits, uint32_t* ws,
                             hipStream_t s, int* result);
 
-// Particle chunk (packed xyz, n < 2^32) -> SoA coordinates in cell-key
-// order in x[0], y[0], z[0] (x[1], y[1], z[1] are scratch), and the cell
-// table: cell_end[c] = number of particles in cells <= c.  Needs g.g <=
+// Particles of a chunk for the cell sort: packed FP32 xyz, or (uv set) a
+// compressed file's uint16 triplets, decoded with gd where the first pass
+// reads them; compressed files of one tile per side only (gd.one_tile).
+struct ParticleSource {
+  const float* xyz = nullptr;
+  const uint16_t* uv = nullptr;
+  GadgetDecode gd{};
+};
+
+// Particle chunk (n < 2^32) -> SoA coordinates in cell-key order in x[0],
+// y[0], z[0] (x[1], y[1], z[1] are scratch), and the cell table:
+// cell_end[c] = number of particles in cells <= c.  Needs g.g <=
 // kCsMaxCells and g.gx <= kCsMaxCellsX.  The order of the particles inside
 // one cell is unspecified.
-hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
-                               float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s);
+hipError_t cell_sort_particles(const ParticleSource& src, uint64_t n, Grid g, float* x[2],
+                               float* y[2], float* z[2], uint32_t* cell_end, uint32_t* ws,
+                               hipStream_t s);
 
 // Particle chunk (packed xyz) -> cell keys + SoA coordinates.
 hipError_t particle_keys(const float* xyz, uint64_t n, Grid g, uint32_t* key,

@@ -428,22 +428,113 @@ __device__ __forceinline__ void xcd_range(uint32_t ntiles, uint32_t& t, uint32_t
   end = (uint32_t)((uint64_t)ntiles * (x + 1) / 8);
 }
 
-// Histogram of tile r's cells of coordinate c[i * stride] (ndig <= kDig
-// cells, 1/inv wide), written to hist[r.hbase + d * r.hstride].
-template <int kDig>
-__device__ __forceinline__ void hist_tile(const float* __restrict__ c, uint32_t stride, float inv,
-                                          int ndig, const TileRange& r, uint32_t* s_h,
-                                          uint32_t* __restrict__ hist) {
+// Particle sources of the sort passes.  A pass loads what its items need
+// first (load / load1<AXIS>: the raw words of particle i) and turns them into
+// coordinates afterwards (finish / finish1<AXIS>), so that every item's loads
+// are in flight before any of them is waited for.
+struct Packed3 {
+  float x, y, z;
+};
+
+struct SrcPacked {  // packed FP32 xyz (the file format)
+  const float* __restrict__ xyz;
+  using Raw = Packed3;
+  using Raw1 = float;
+  __device__ Raw load(uint64_t i) const { return reinterpret_cast<const Packed3*>(xyz)[i]; }
+  __device__ void finish(uint64_t, const Raw& w, float& x, float& y, float& z) const {
+    x = w.x;
+    y = w.y;
+    z = w.z;
+  }
+  template <int AXIS>
+  __device__ Raw1 load1(uint64_t i) const {
+    return xyz[3 * i + AXIS];
+  }
+  template <int AXIS>
+  __device__ float finish1(uint64_t, Raw1 w) const {
+    return w;
+  }
+};
+
+struct SrcSoA {  // the sort's SoA buffers
+  const float* __restrict__ x;
+  const float* __restrict__ y;
+  const float* __restrict__ z;
+  using Raw = Packed3;
+  using Raw1 = float;
+  __device__ Raw load(uint64_t i) const { return Packed3{x[i], y[i], z[i]}; }
+  __device__ void finish(uint64_t, const Raw& w, float& px, float& py, float& pz) const {
+    px = w.x;
+    py = w.y;
+    pz = w.z;
+  }
+  template <int AXIS>
+  __device__ Raw1 load1(uint64_t i) const {
+    return AXIS == 0 ? x[i] : (AXIS == 1 ? y[i] : z[i]);
+  }
+  template <int AXIS>
+  __device__ float finish1(uint64_t, Raw1 w) const {
+    return w;
+  }
+};
+
+// A compressed file's uint16 triplets, decoded where they are read (the
+// decode is exact and the same wherever it runs).  Files of one tile per
+// side only: the general decode holds too many registers per item for the
+// scatter to keep all of a thread's items in flight.
+struct SrcGadget {
+  const uint16_t* __restrict__ uv;
+  GadgetDecode g;
+  struct Raw {
+    uint32_t d[3];
+  };
+  using Raw1 = uint32_t;
+  __device__ Raw load(uint64_t i) const {
+    return Raw{{uv[3 * i], uv[3 * i + 1], uv[3 * i + 2]}};
+  }
+  __device__ void finish(uint64_t i, const Raw& w, float& x, float& y, float& z) const {
+    float p[3];
+    gadget_point<true>(g, i, w.d, p);
+    x = p[0];
+    y = p[1];
+    z = p[2];
+  }
+  template <int AXIS>
+  __device__ Raw1 load1(uint64_t i) const {
+    return uv[3 * i + AXIS];
+  }
+  template <int AXIS>
+  __device__ float finish1(uint64_t i, Raw1 w) const {
+    int ip[3];
+    gadget_grid<true>(g, i, ip);
+    return gadget_coord(g, ip[AXIS], w);
+  }
+};
+
+// Histogram of tile r's cells of coordinate AXIS of src (ndig <= kDig cells,
+// 1/inv wide), written to hist[r.hbase + d * r.hstride].
+template <int kDig, int AXIS, class Src>
+__device__ __forceinline__ void hist_tile(const Src& src, float inv, int ndig, const TileRange& r,
+                                          uint32_t* s_h, uint32_t* __restrict__ hist) {
   const int tid = threadIdx.x;
   for (int i = tid; i < kDig; i += kCsThreads) s_h[i] = 0;
   __syncthreads();
-  const float* cb = c + r.begin * stride;
+  typename Src::Raw1 w[kCsItems];
+  if (r.len == (uint32_t)kCsTile) {  // full tile: no guard between the loads
+#pragma unroll
+    for (int j = 0; j < kCsItems; ++j) w[j] = src.template load1<AXIS>(r.begin + j * kCsThreads + tid);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kCsItems; ++j) {
+      const uint32_t i = (uint32_t)(j * kCsThreads + tid);
+      w[j] = typename Src::Raw1{};
+      if (i < r.len) w[j] = src.template load1<AXIS>(r.begin + i);
+    }
+  }
   float v[kCsItems];
 #pragma unroll
-  for (int j = 0; j < kCsItems; ++j) {
-    const uint32_t i = (uint32_t)(j * kCsThreads + tid);
-    v[j] = i < r.len ? cb[(uint64_t)i * stride] : 0.0f;
-  }
+  for (int j = 0; j < kCsItems; ++j)
+    v[j] = src.template finish1<AXIS>(r.begin + (uint32_t)(j * kCsThreads + tid), w[j]);
 #pragma unroll
   for (int j = 0; j < kCsItems; ++j) {
     const bool ok = (uint32_t)(j * kCsThreads + tid) < r.len;
@@ -453,21 +544,17 @@ __device__ __forceinline__ void hist_tile(const float* __restrict__ c, uint32_t 
   for (int d = tid; d < ndig; d += kCsThreads) hist[r.hbase + (uint64_t)d * r.hstride] = s_h[d];
 }
 
-// Per-tile histogram of the cells of coordinate c[i * stride] (passes 1, 2).
-template <bool SLAB>
+// Per-tile histogram of the cells of coordinate AXIS (passes 1, 2).
+template <int AXIS, bool SLAB, class Src>
 __global__ __launch_bounds__(kCsThreads) void cs_hist_kernel(
-    const float* __restrict__ c, uint32_t stride, uint64_t n, float inv, int G, uint32_t ntiles,
-    const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb, uint32_t* __restrict__ hist) {
+    Src src, uint64_t n, float inv, int G, uint32_t ntiles, const uint32_t* __restrict__ ss,
+    const uint32_t* __restrict__ tb, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[kCsMaxCells];
   TileRange r;
   const uint32_t t = xcd_tile();
   if (!(SLAB ? tile_range_slab(t, ss, tb, G, r) : tile_range_chunk(t, ntiles, n, r))) return;
-  hist_tile<kCsMaxCells>(c, stride, inv, G, r, s_h, hist);
+  hist_tile<kCsMaxCells, AXIS>(src, inv, G, r, s_h, hist);
 }
-
-struct Packed3 {
-  float x, y, z;
-};
 
 // Scatter of tile r by the cell of coordinate AXIS (ndig <= kDig cells, 1/inv
 // wide).  Without `lt` the digit runs go to the offsets of the scanned
@@ -476,9 +563,9 @@ struct Packed3 {
 // plus the row's particles of lower digits, plus the digit's particles in
 // the row's earlier tiles; the row's first tile also writes the row's cell
 // table.  Input packed xyz (AOS) or SoA; output SoA.
-template <int AXIS, bool AOS, int kDig>
+template <int AXIS, int kDig, class Src>
 __device__ __forceinline__ void scatter_tile(
-    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    const Src& src,
     float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float inv, int ndig,
     const TileRange& r, const uint32_t* __restrict__ hist, const LongTile* lt,
     uint32_t* __restrict__ cell_end, float (*s_c)[kCsTile], uint32_t* s_cnt, uint32_t* s_off,
@@ -489,24 +576,22 @@ __device__ __forceinline__ void scatter_tile(
   const int tid = threadIdx.x;
   for (int i = tid; i < kDig; i += kCsThreads) s_cnt[i] = 0;
 
-  float px[kCsItems], py[kCsItems], pz[kCsItems];
+  typename Src::Raw w[kCsItems];
+  if (r.len == (uint32_t)kCsTile) {  // full tile: no guard between the loads
 #pragma unroll
-  for (int j = 0; j < kCsItems; ++j) {
-    const uint32_t i = (uint32_t)(j * kCsThreads + tid);
-    px[j] = py[j] = pz[j] = 0.0f;
-    if (i < r.len) {
-      if (AOS) {
-        const Packed3 p = reinterpret_cast<const Packed3*>(ix)[r.begin + i];
-        px[j] = p.x;
-        py[j] = p.y;
-        pz[j] = p.z;
-      } else {
-        px[j] = ix[r.begin + i];
-        py[j] = iy[r.begin + i];
-        pz[j] = iz[r.begin + i];
-      }
+    for (int j = 0; j < kCsItems; ++j) w[j] = src.load(r.begin + j * kCsThreads + tid);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kCsItems; ++j) {
+      const uint32_t i = (uint32_t)(j * kCsThreads + tid);
+      w[j] = typename Src::Raw{};
+      if (i < r.len) w[j] = src.load(r.begin + i);
     }
   }
+  float px[kCsItems], py[kCsItems], pz[kCsItems];
+#pragma unroll
+  for (int j = 0; j < kCsItems; ++j)
+    src.finish(r.begin + (uint32_t)(j * kCsThreads + tid), w[j], px[j], py[j], pz[j]);
   __syncthreads();
 
   uint32_t rk[kCsItems];  // digit << 16 | slot among the tile's items of that digit
@@ -598,9 +683,9 @@ __device__ __forceinline__ void scatter_tile(
 
 // Scatter of one tile of passes 1 (by cz, chunk tiles) and 2 (by cy, slab
 // tiles).
-template <int AXIS, bool AOS, bool SLAB>
+template <int AXIS, bool SLAB, class Src>
 __global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
-    const float* __restrict__ ix, const float* __restrict__ iy, const float* __restrict__ iz,
+    Src src,
     float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, uint64_t n, Grid g,
     uint32_t ntiles, const uint32_t* __restrict__ ss, const uint32_t* __restrict__ tb,
     const uint32_t* __restrict__ hist) {
@@ -611,8 +696,8 @@ __global__ __launch_bounds__(kCsThreads) void cs_scatter_kernel(
   TileRange r;
   const uint32_t t = xcd_tile();
   if (!(SLAB ? tile_range_slab(t, ss, tb, g.g, r) : tile_range_chunk(t, ntiles, n, r))) return;
-  scatter_tile<AXIS, AOS, kCsMaxCells>(ix, iy, iz, ox, oy, oz, g.inv_cell, g.g, r, hist, nullptr,
-                                       nullptr, s_c, s_cnt, s_off, s_tmp);
+  scatter_tile<AXIS, kCsMaxCells>(src, ox, oy, oz, g.inv_cell, g.g, r, hist, nullptr, nullptr, s_c,
+                                  s_cnt, s_off, s_tmp);
 }
 
 // Slab starts and the slab-major tiling of pass 2, from pass 1's scanned
@@ -765,7 +850,7 @@ __global__ __launch_bounds__(kCsThreads) void cs_long_hist_kernel(
   for (; t < end; t += step) {
     const TileRange r = long_range(long_tile(t, lrows, trow), g.gx);
     __syncthreads();  // the previous tile's readers of s_h are done
-    hist_tile<kCsMaxCellsX>(x, 1u, g.inv_cell_x, g.gx, r, s_h, hist3);
+    hist_tile<kCsMaxCellsX, 0>(SrcSoA{x, x, x}, g.inv_cell_x, g.gx, r, s_h, hist3);
   }
 }
 
@@ -787,7 +872,7 @@ __global__ __launch_bounds__(kCsThreads) void cs_long_scatter_kernel(
     const LongTile lt = long_tile(t, lrows, trow);
     const TileRange r = long_range(lt, g.gx);
     __syncthreads();  // the previous tile's readers of the LDS are done
-    scatter_tile<0, false, kCsMaxCellsX>(ix, iy, iz, ox, oy, oz, g.inv_cell_x, g.gx, r, hist3, &lt,
+    scatter_tile<0, kCsMaxCellsX>(SrcSoA{ix, iy, iz}, ox, oy, oz, g.inv_cell_x, g.gx, r, hist3, &lt,
                                          cell_end, s_c, s_cnt, s_off, s_tmp);
   }
 }
@@ -910,11 +995,12 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* pay[2][4], int npay, ui
   return hipSuccess;
 }
 
-hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2], float* y[2],
-                               float* z[2], uint32_t* cell_end, uint32_t* ws, hipStream_t s) {
+hipError_t cell_sort_particles(const ParticleSource& src, uint64_t n, Grid g, float* x[2],
+                               float* y[2], float* z[2], uint32_t* cell_end, uint32_t* ws,
+                               hipStream_t s) {
   const int G = g.g;
   if (G > kCsMaxCells || G < 1 || g.gx > kCsMaxCellsX || g.gx < 1) return hipErrorInvalidValue;
-  if (n > 0xffffffffull) return hipErrorInvalidValue;
+  if (n > 0xffffffffull || (src.uv && !src.gd.one_tile)) return hipErrorInvalidValue;
   if (n == 0) return fill_u32(cell_end, 0u, g.ncells, s);
   const uint64_t tiles1 = ceil_div(n, kCsTile);
   const uint64_t tiles2 = tiles1 + (uint64_t)G;  // each slab adds at most one partial tile
@@ -934,20 +1020,30 @@ hipError_t cell_sort_particles(const float* xyz, uint64_t n, Grid g, float* x[2]
   const uint32_t nb2 = (uint32_t)(8 * ceil_div(tiles2, 8));
   const uint32_t nrows = (uint32_t)(G * G);
   const uint32_t nb3 = (uint32_t)(8 * ceil_div(nrows, 8));
-  // pass 1: by cz, packed xyz -> buffers 0
-  cs_hist_kernel<false><<<nb1, kCsThreads, 0, s>>>(xyz + 2, 3u, n, g.inv_cell, G, nt1, nullptr,
-                                                    nullptr, hist1);
-  hipError_t e = exclusive_sum_u32(hist1, hist1, (uint64_t)G * tiles1, scan_ws, s);
+  // pass 1: by cz, packed xyz (or compressed triplets decoded as they are
+  // read) -> buffers 0
+  hipError_t e = hipSuccess;
+  auto pass1 = [&](auto in) {
+    cs_hist_kernel<2, false><<<nb1, kCsThreads, 0, s>>>(in, n, g.inv_cell, G, nt1, nullptr,
+                                                         nullptr, hist1);
+    e = exclusive_sum_u32(hist1, hist1, (uint64_t)G * tiles1, scan_ws, s);
+    if (e == hipSuccess)
+      cs_scatter_kernel<2, false><<<nb1, kCsThreads, 0, s>>>(in, x[0], y[0], z[0], n, g, nt1,
+                                                              nullptr, nullptr, hist1);
+  };
+  if (!src.uv)
+    pass1(SrcPacked{src.xyz});
+  else
+    pass1(SrcGadget{src.uv, src.gd});
   if (e != hipSuccess) return e;
-  cs_scatter_kernel<2, true, false><<<nb1, kCsThreads, 0, s>>>(
-      xyz, nullptr, nullptr, x[0], y[0], z[0], n, g, nt1, nullptr, nullptr, hist1);
   cs_slab_tiles_kernel<<<1, kCsMaxCells, 0, s>>>(hist1, nt1, n, G, ss, tb, lctr);
   // pass 2: by cy inside each slab, buffers 0 -> 1
-  cs_hist_kernel<true><<<nb2, kCsThreads, 0, s>>>(y[0], 1u, n, g.inv_cell, G, 0u, ss, tb, hist2);
+  cs_hist_kernel<1, true><<<nb2, kCsThreads, 0, s>>>(SrcSoA{x[0], y[0], z[0]}, n, g.inv_cell, G,
+                                                      0u, ss, tb, hist2);
   e = exclusive_sum_u32(hist2, hist2, (uint64_t)G * tiles2, scan_ws, s);
   if (e != hipSuccess) return e;
-  cs_scatter_kernel<1, false, true><<<nb2, kCsThreads, 0, s>>>(
-      x[0], y[0], z[0], x[1], y[1], z[1], n, g, 0u, ss, tb, hist2);
+  cs_scatter_kernel<1, true><<<nb2, kCsThreads, 0, s>>>(SrcSoA{x[0], y[0], z[0]}, x[1], y[1], z[1],
+                                                         n, g, 0u, ss, tb, hist2);
   // pass 3: by cx inside each row, buffers 1 -> 0, and the cell table; rows
   // of more than kCsRowCap particles in kCsTile tiles, like passes 1 and 2
   cs_row_kernel<<<nb3, kCsRowThreads, 0, s>>>(x[1], y[1], z[1], x[0], y[0], z[0], cell_end, n, g,

@@ -782,6 +782,41 @@ def test_compressed_chunks_equal_decoded_chunks(built, eng):
     assert int(got.sum()) > 0
 
 
+SORT_DECODE_SETS = dict(DECODE_SETS, **{
+    # one tile, ids mid-grid, partial sort tiles (the cell sort's own decode)
+    "grid200": (200 ** 3, 1000.0, -4.0, 4.0, 16, 5_000_001, 1, 301_111),
+})
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("name", sorted(SORT_DECODE_SETS))
+def test_compressed_sort_equals_decoded(built, eng, name, fused):
+    """Files of one tile per side are decoded by the cell sort's first pass
+    as it reads them, the others by the decode kernel before the sort: both
+    count what the oracle's decode of the same triplets counts, with halos
+    next to particles so that the coordinates' low bits matter."""
+    total, box, dmin, dmax, bits, id0, ntile, n = SORT_DECODE_SETS[name]
+    n = min(n, 400_000)
+    prm = gadget_params(built, total, box, dmin, dmax, bits, id0, ntile, fused)
+    rng = np.random.default_rng(900 + 2 * len(name) + fused)
+    uv = rng.integers(0, 1 << bits, (n, 3), dtype=np.uint16)
+    p = t.oracle_gadget_decode(uv, as_dict(prm), fused)
+    # halos where the particles are, also outside the box (ids below 1 put
+    # the particles at negative coordinates)
+    h = (p[rng.integers(0, n, 4000)] + rng.normal(0.0, 0.05, (4000, 3))).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.001, 2.0, 16)
+    print("sort decode", name, "fused", fused, "n", n, flush=True)
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    eng.add_compressed_particles(uv, prm)
+    got = eng.counts()
+    eng.reset_counts()
+    eng.add_particles(p)
+    want = eng.counts()
+    assert_counts_equal(got, want, h)
+    assert int(got.sum()) > 0
+
+
 def test_compressed_device_chunks(built, eng):
     """Triplets already on the device give what host triplets give."""
     rng = np.random.default_rng(78)
