@@ -145,14 +145,18 @@ int sjcs_gadget_decode_params(const sjcs_gadget_header *hdr, int ntile,
 
 /* One compressed particle file's triplets (uv[3n], host) as one particle
  * chunk: uploaded as 6 bytes per particle, decoded on the device, then
- * indexed and counted as sjcs_add_particles does.  Replaces the compressed
+ * indexed and counted as sjcs_add_particles does.  A file of one tile per
+ * side is decoded by the index build's first pass as it reads the triplets
+ * (no decoded copy is stored); other files, and the packed-key sort's input,
+ * go through a decode into device staging first.  Replaces the compressed
  * branch of constructIndexWithSameThreadPool (astr_rcount/main.c:739-763).
  * SJCS_ERR_INVALID when an id of the file is out of the range the decode
  * handles exactly (grid points beyond +-2^31, |id| >= 2^50). */
 int sjcs_add_compressed_particles(sjcs_engine *e, const uint16_t *uv, uint64_t n,
                                   const sjcs_gadget_decode *p);
 /* The same with the triplets already on the device (d_uv, 3n uint16); the
- * upload phase of sjcs_last_chunk_timings is then the decode alone. */
+ * upload phase of sjcs_last_chunk_timings is then empty, and a decode that
+ * runs before the build (files of several tiles per side) is in no phase. */
 int sjcs_add_compressed_particles_device(sjcs_engine *e, const uint16_t *d_uv, uint64_t n,
                                          const sjcs_gadget_decode *p);
 
