@@ -205,6 +205,9 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *                       waits)
  *   "index_sort"        0 = automatic (axis by axis up to 256 cells per
  *                       axis), 1 = always by the packed cell key
+ *   "piece_max"         most particles of a chunk indexed and counted as
+ *                       one piece (0 = the most that 32-bit offsets and
+ *                       counts allow; small values exercise the split)
  * Returns SJCS_ERR_INVALID for an unknown key or value. */
 int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
 

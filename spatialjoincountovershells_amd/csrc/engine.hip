@@ -245,6 +245,7 @@ struct sjcs_engine {
   int64_t x_split = 0;
   int64_t overlap = 0;
   int64_t fold_limit = 0;  // 0: fold before a u32 count could pass 2^32 - 1
+  int64_t piece_max = 0;   // 0: pieces as large as 32-bit offsets and counts allow
   int64_t graph = 0;       // 1: each piece's sort + count replayed from a graph
   int64_t tiles = 1;       // 1: halo tiles share their candidates (ascending shells)
 
@@ -864,7 +865,8 @@ int piece_done(sjcs_engine* e, uint64_t n) {
 // A chunk, in pieces small enough for 32-bit particle offsets and for
 // per-piece counts below 2^32.
 int add_chunk(sjcs_engine* e, const ParticleSource& src, uint64_t n) {
-  const uint64_t cap = std::min<uint64_t>(0x3fffffffull, 0xffffffffull / images_max(e));
+  uint64_t cap = std::min<uint64_t>(0x3fffffffull, 0xffffffffull / images_max(e));
+  if (e->piece_max > 0) cap = std::min<uint64_t>(cap, (uint64_t)e->piece_max);
   uint64_t done = 0;
   do {
     const uint64_t m = std::min(cap, n - done);
@@ -1199,7 +1201,8 @@ int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
               {"kernel", e->kernel},                   {"lanes_per_halo", e->lanes_per_halo},
               {"segment", e->segment},                 {"tiles", e->tiles},
               {"graph", e->graph},                     {"fold_limit", e->fold_limit},
-              {"overlap", e->overlap},                 {"index_sort", e->index_sort}};
+              {"overlap", e->overlap},                 {"index_sort", e->index_sort},
+              {"piece_max", e->piece_max}};
   for (const auto& o : opts)
     if (strcmp(key, o.k) == 0) {
       *value = o.v;
@@ -1331,6 +1334,11 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
   if (strcmp(key, "fold_limit") == 0) {
     if (value < 0 || value > 0xffffffffll) return fail(e, SJCS_ERR_INVALID, "fold_limit: 0 to 2^32-1");
     e->fold_limit = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "piece_max") == 0) {
+    if (value < 0) return fail(e, SJCS_ERR_INVALID, "piece_max: 0 or more");
+    e->piece_max = value;
     return SJCS_OK;
   }
   if (strcmp(key, "overlap") == 0) {

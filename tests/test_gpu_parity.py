@@ -843,6 +843,41 @@ def test_compressed_device_chunks(built, eng):
     assert int(want.sum()) > 0
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_pieces_equal_whole_chunks(built, eng, graph):
+    """A chunk cut into pieces (piece_max, an odd size, so pieces end inside
+    sort tiles) counts what it counts whole: packed particles, a compressed
+    file of one tile per side (each piece decoded by its sort from its own
+    first id) and one of three tiles per side (decoded before the sort)."""
+    rng = np.random.default_rng(4242)
+    _, r2 = t.oracle_thresholds(0.001, 4.0, 12)
+    p, h = gen_uniform(43, 250_000, 3_000)
+    h = np.vstack([h, p[:1500] + np.float32(0.5)]).astype(np.float32)
+    one = gadget_params(built, 100 ** 3, 1000.0, -4.0, 4.0, 16, 1)
+    many = gadget_params(built, 300 ** 3, 900.0, -3.0, 1.0, 12, 1, 3)
+    uv1 = rng.integers(0, 1 << 16, (300_000, 3), dtype=np.uint16)
+    uv3 = rng.integers(0, 1 << 12, (300_000, 3), dtype=np.uint16)
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    res = []
+    try:
+        eng.set_option("graph", graph)
+        for cap in (0, 70_001):
+            print("pieces graph", graph, "piece_max", cap, flush=True)
+            eng.set_option("piece_max", cap)
+            assert eng.get_option("piece_max") == cap
+            eng.reset_counts()
+            eng.add_particles(p)
+            eng.add_compressed_particles(uv1, one)
+            eng.add_compressed_particles(uv3, many)
+            res.append(eng.counts())
+    finally:
+        eng.set_option("piece_max", 0)
+        eng.set_option("graph", 0)
+    assert_counts_equal(res[1], res[0], h)
+    assert int(res[0].sum()) > 0
+
+
 def synth_snapshot(rng, side):
     from spatialjoincountovershells_amd import synth
     return synth.gadget_snapshot(rng, side)
