@@ -59,7 +59,7 @@ CONFIGS = {
                         "power-law sizes) x 5e6 halos (80% at clump centres), "
                         "--radius 0.01:20:64, one chunk",
                np=200_000_000, nh=5_000_000, radius=(0.01, 20.0, 64), chunks=1,
-               clustered=True, autotune=True),
+               clustered=True, autotune=True, compare_plain=True),
     "c5": dict(workload="serving: 3e8 uniform particles resident; requests of floor(10^U[0,5]) "
                         "uniform halos, K ~ U{8..64}, r_max ~ U[1,10], r_min = r_max/5000; "
                         "Poisson arrivals",
@@ -86,6 +86,40 @@ METRICS = {
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def csrc_digest():
+    """sha256 of the HIP sources (csrc/): counter figures in profiles/*.json
+    carry the digest of the sources they were measured on."""
+    import hashlib
+    d = os.path.join(ROOT, "spatialjoincountovershells_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def measured_profile(name):
+    """profiles/<name> when it was measured on the current sources, else
+    None (a stale counter figure is not reported)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        js = json.load(f)
+    if js.get("csrc_sha256") != csrc_digest():
+        log(f"[bench] {name}: measured on other sources, not reported")
+        return None
+    return js
+
+
+def host_threads():
+    """Host cores this process may run on (the CPU baseline's thread count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def make_inputs(cfg, seed):
@@ -345,7 +379,12 @@ def main():
                     help="sort the next chunk while this one is counted")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-halos", type=int, default=1_000_000)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline and the CLI (0 = every core "
+                         "this process may use)")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="c4: timed steps of the plain kernel (one halo per wavefront, no "
+                         "tiles, default cells) after the tuned ones; 0 = skip")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--requests", type=int, default=2000, help="serving config: requests")
     ap.add_argument("--wall", action="store_true",
@@ -371,6 +410,8 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_threads <= 0:
+        args.cpu_threads = host_threads()
     cfg = CONFIGS[args.config]
     if cfg.get("serving"):
         return serve_bench(args, cfg)
@@ -392,7 +433,10 @@ def main():
     if args.graph or cfg.get("graph"):
         eng.set_option("graph", 1)
     eng.set_shells(r2)
+    t_h = time.perf_counter()
     eng.set_halos(halos)
+    halo_wall_ms = (time.perf_counter() - t_h) * 1e3
+    halo_sort_ms = eng.phase_timings()["halo_sort"]
     d_parts = [eng.device_array(p[1] if is_compressed(p) else p) for p in parts]
     n = cfg["np"]
 
@@ -413,11 +457,12 @@ def main():
         log(f"[bench] autotune: {tuned}")
 
     # exact W of one step: an untimed pair-test pass per chunk
-    w = 0
+    w_chunks = []
     for d, p in zip(d_parts, parts):
         eng.reset_counts()
         add_device_chunk(eng, d, p)
-        w += eng.last_chunk_pair_tests()
+        w_chunks.append(eng.last_chunk_pair_tests())
+    w = sum(w_chunks)
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -435,11 +480,48 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1000.0 / args.steps
 
+    t_r = time.perf_counter()
     counted = int(eng.counts().sum())
+    counts_wall_ms = (time.perf_counter() - t_r) * 1e3
+    readout_ms = eng.phase_timings()["readout"]
     build_ms = float(phase[:, 1].mean())
     count_ms = float(phase[:, 2].mean())
+    graph = bool(args.graph or cfg.get("graph"))
     log(f"[bench] W={w} counted={counted} step={ms_step:.3f} ms build={build_ms:.3f} "
         f"count={count_ms:.3f} ms (last chunk)")
+
+    plain = None
+    if cfg.get("compare_plain") and not args.plain and args.plain_steps > 0:
+        # the plain one-halo-per-wavefront kernel on the same inputs, same
+        # run: default cells, no tiles
+        tuned_opts = {k_: eng.get_option(k_) for k_ in
+                      ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles")}
+        for k_ in ("cell_size_milli", "x_split", "lanes_per_halo", "segment"):
+            eng.set_option(k_, 0)
+        eng.set_option("tiles", 0)
+        eng.set_option("kernel", 1)
+        log(f"[bench] plain kernel: {args.plain_steps} steps")
+        step()
+        eng.sync()
+        pph = np.zeros((args.plain_steps, 3))
+        t0 = time.perf_counter()
+        for s_ in range(args.plain_steps):
+            step()
+            pph[s_] = eng.last_chunk_timings()
+        eng.sync()
+        p_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+        p_counted = int(eng.counts().sum())
+        plain = {"ms_per_step": round(p_ms, 3), "count_ms": round(float(pph[:, 2].mean()), 3),
+                 "index_build_ms": round(float(pph[:, 1].mean()), 3),
+                 "pair_tests_per_s": w / (p_ms * 1e-3),
+                 "same_counts_total": p_counted == counted,
+                 "step_speedup_tuned_vs_plain": round(p_ms / ms_step, 3),
+                 "count_speedup_tuned_vs_plain": round(float(pph[:, 2].mean()) / count_ms, 3),
+                 "target": ">= 2x (batched persistent kernel vs plain)"}
+        eng.set_option("kernel", 0)
+        for k_, v_ in tuned_opts.items():
+            eng.set_option(k_, v_)
+        log(f"[bench] plain: {plain}")
 
     nh = halos.shape[0]
     # algorithmic bytes of one count launch: particles read once, halo SoA
@@ -447,22 +529,22 @@ def main():
     # later chunk adds into them)
     count_bytes = 12 * n + 12 * nh + 4 * k * nh * (2 if cfg["chunks"] > 1 else 1)
     count_gbs = count_bytes / (count_ms * 1e-3) / 1e9
-    count_tflops = 8 * w / (count_ms * 1e-3) / 1e12
+    count_tflops = 8 * w_chunks[-1] / (count_ms * 1e-3) / 1e12
     # index build: packed xyz (12 B) or compressed triplets (6 B) read once,
     # sorted SoA xyz written once
     build_bytes = (6 if cfg.get("compressed") else 12) * n + 12 * n
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     dominant = "count" if count_ms >= build_ms else "index_build"
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tj = json.load(f)
+    tj = measured_profile("traffic.json")
+    if tj is not None:
         traffic = tj.get(args.config, {}).get(dominant)
     if dominant == "count":
         roof = {"bound": "hbm", "achieved": round(count_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(count_gbs / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "kernel": "count_group_kernel",
+                "traffic": traffic,
+                "kernel": "count_group_kernel" + (" (graph replay: timed together with the "
+                                                  "cell sort)" if graph else ""),
                 "bytes_per_launch": count_bytes}
     else:
         roof = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
@@ -471,11 +553,8 @@ def main():
                 "bytes_per_launch": build_bytes}
 
     # measured VALU issue of the count kernel (rocprofv3 counters, profiles/)
-    vfile = os.path.join(ROOT, "profiles", "valu.json")
-    valu = None
-    if os.path.exists(vfile):
-        with open(vfile) as f:
-            valu = json.load(f).get(args.config, {}).get("count")
+    vj = measured_profile("valu.json")
+    valu = vj.get(args.config, {}).get("count") if vj is not None else None
 
     cpu = None
     if not args.no_cpu_baseline and args.config in ("c2", "c3", "c4", "c2z"):
@@ -509,11 +588,21 @@ def main():
                    "chunks": cfg["chunks"], "halos": nh, "shells": k,
                    "radius": f"{mn}:{mx}:{k}", "pair_tests_per_step": w,
                    "pairs_counted": counted, "parallelism": "single GPU"},
-        "phases_ms": {"index_build": round(build_ms, 4), "count": round(count_ms, 4),
-                      "of": ("the last chunk of a step" if cfg["chunks"] > 1 else "the chunk") +
-                            (" (graph replay: sort and count timed together as count)"
-                             if args.graph or cfg.get("graph") else "")},
-        "count_kernel": {"pair_tests_per_s": w / (count_ms * 1e-3),
+        "phases_ms": {
+            **({"index_build+count (graph replay)": round(count_ms, 4)} if graph else
+               {"index_build": round(build_ms, 4), "count": round(count_ms, 4)}),
+            "of": "the last chunk of a step" if cfg["chunks"] > 1 else "the chunk",
+            "outside_the_step": {
+                "halo_sort_device": round(halo_sort_ms, 4),
+                "halo_upload_and_sort_wall": round(halo_wall_ms, 3),
+                "readout_device": round(readout_ms, 4),
+                "readout_and_download_wall": round(counts_wall_ms, 3),
+                "note": "once per halo set / per result, not per step: the halos are "
+                        "sorted before the chunk loop, the counts put back in input "
+                        "order and copied to the host after it"}},
+        "count_kernel": {"of": "the last chunk of a step" + (" (graph replay: its sort "
+                                                              "included)" if graph else ""),
+                         "pair_tests_per_s": w_chunks[-1] / (count_ms * 1e-3),
                          "valu_tflops_8_per_test": round(count_tflops, 3),
                          "valu_frac_of_fp32_peak": round(count_tflops / FP32_PEAK_TFLOPS, 4),
                          "hbm_gbs_algorithmic": round(count_gbs, 1),
@@ -521,6 +610,7 @@ def main():
                          "valu_lane_slots_per_pair_test":
                              valu and valu.get("valu_lane_slots_per_pair_test")},
         "autotune": tuned,
+        "plain_kernel": plain,
         "wall": wall,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -178,7 +178,9 @@ int sjcs_get_counts_range(sjcs_engine *e, uint64_t first, uint64_t nhalos, uint6
 
 /* Exact number of in-box (halo image, particle) pair tests the last chunk
  * made -- the reference's CalcDistance calls (MultiRangeCount.c:107).  Runs
- * one extra untimed pass; a measurement aid, not part of the count. */
+ * one extra untimed pass; a measurement aid, not part of the count.  When
+ * "piece_max" (or the 2^30-1 particle limit of one piece) cuts a chunk into
+ * pieces, this is the last piece's figure. */
 int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
 
 /* Tuning knobs (no reference counterpart).  Keys:
@@ -224,8 +226,16 @@ int sjcs_get_option(sjcs_engine *e, const char *key, int64_t *value);
 int sjcs_autotune(sjcs_engine *e, const float *d_xyz, uint64_t n, float *best_ms);
 
 /* Milliseconds the device spent on the last chunk, by phase (HIP events):
- * [0] upload, [1] index build (key + sort + cell table), [2] count. */
+ * [0] upload, [1] index build (key + sort + cell table), [2] count.  For a
+ * chunk cut into pieces, [1] and [2] are the last piece's. */
 int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);
+
+/* The three phases above, then [3] the last halo preparation on the device
+ * (sjcs_set_halos*: keys, Morton sort, tiling; host steps between them
+ * included) and [4] the last read-out of the counts into input row order
+ * (the first sjcs_get_counts* after the totals changed).  No reference
+ * counterpart: the reference's own timers are the CLI's >>>> lines. */
+int sjcs_phase_timings(sjcs_engine *e, float ms[5]);
 
 /* ---- Serving loop (no reference counterpart) ----
  * Requests -- a batch of halo centres with its own squared shell thresholds
@@ -279,6 +289,7 @@ void sjcs_host_free(void *p);
 
 /* ---- index: the ArraySTR lifecycle, one device-resident particle index ---- */
 
+/* ndata: at most 2^30-1 particles. */
 int sjcs_index_create(sjcs_engine *e, uint64_t ndata, sjcs_index **out);
 int sjcs_index_insert(sjcs_index *ix, const float *xyz, uint64_t n);
 int sjcs_index_build(sjcs_index *ix);

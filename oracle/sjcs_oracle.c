@@ -11,7 +11,9 @@
  * its result is the set definition of the count, independent of any index
  * structure (the reference's STR tree only prunes; internal MBRs are exact
  * min/max of their children, src/objects/ArraySTR/Node.c:21-31, and the box
- * test is monotone, so pruning never changes which leaves pass).
+ * test is monotone, so pruning never changes which leaves pass).  The
+ * triples are visited in cache-sized blocks of particles per group of halos,
+ * each one once.
  *
  * Reference behaviour restated, with the file:line each step follows:
  *   thresholds   include/support_functions.h:18-34 (Logspace), with the
@@ -41,7 +43,7 @@
  *                fused into fmaf() as gcc -std=gnu99 contracts it for an FMA
  *                host, or left as two roundings (fused = 0).
  *
- * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  The
+ * Build: gcc -O3 -ffp-contract=off -fPIC -shared (oracle/Makefile).  The
  * -ffp-contract=off is what keeps the distance free of FMAs.
  */
 #include <math.h>
@@ -157,31 +159,56 @@ static int in_box(const float p[3], const float lo[3], const float hi[3])
     return 1;
 }
 
-static void count_one_halo(const float *particles, uint64_t np,
-                           const float c[3], const float *r2, int nrads,
-                           float rbox, uint64_t *row, uint64_t *inbox)
-{
-    float centres[8][3];
-    int nimg = oracle_images(c, rbox, centres);
-    int img, d;
-    uint64_t i;
+/* One query (a halo's periodic image): its centre, closed box and row. */
+typedef struct {
+    float c[3], lo[3], hi[3];
+    uint64_t *row;
+} oracle_query;
 
-    for (img = 0; img < nimg; img++) {
-        float lo[3], hi[3];
-        for (d = 0; d < 3; d++) {
-            lo[d] = centres[img][d] - rbox;
-            hi[d] = centres[img][d] + rbox;
+/* Every (query, particle) pair of a group of queries: the particles are
+ * walked in blocks that stay in cache while every query of the group tests
+ * them, so the particle array is read once per group rather than once per
+ * query.  Each pair is still tested exactly once, so the counts are the
+ * same sums as one query at a time. */
+#define ORACLE_BLOCK 4096
+static void count_queries(const float *particles, uint64_t np,
+                          const oracle_query *q, int nq, const float *r2,
+                          int nrads, uint64_t *inbox)
+{
+    float bx[ORACLE_BLOCK], by[ORACLE_BLOCK], bz[ORACLE_BLOCK];
+    unsigned char pass[ORACLE_BLOCK];
+    uint64_t b0;
+    int j, i, m;
+
+    for (b0 = 0; b0 < np; b0 += ORACLE_BLOCK) {
+        m = (int)(b0 + ORACLE_BLOCK < np ? ORACLE_BLOCK : np - b0);
+        for (i = 0; i < m; i++) {
+            bx[i] = particles[3 * (b0 + i)];
+            by[i] = particles[3 * (b0 + i) + 1];
+            bz[i] = particles[3 * (b0 + i) + 2];
         }
-        for (i = 0; i < np; i++) {
-            const float *p = &particles[3 * i];
-            int k;
-            if (!in_box(p, lo, hi))
-                continue;
-            if (inbox)
-                (*inbox)++;
-            k = oracle_shell_of(oracle_distance(p, centres[img]), r2, nrads);
-            if (k >= 0 && row)
-                row[k]++;
+        for (j = 0; j < nq; j++) {
+            const float lo = q[j].lo[0], hi = q[j].hi[0];
+            /* the x half of in_box for the whole block (a pre-filter: the
+             * full test below decides) */
+            for (i = 0; i < m; i++)
+                pass[i] = (unsigned char)(!(bx[i] < lo) & !(hi < bx[i]));
+            for (i = 0; i < m; i++) {
+                float p[3];
+                int k;
+                if (!pass[i])
+                    continue;
+                p[0] = bx[i];
+                p[1] = by[i];
+                p[2] = bz[i];
+                if (!in_box(p, q[j].lo, q[j].hi))
+                    continue;
+                if (inbox)
+                    (*inbox)++;
+                k = oracle_shell_of(oracle_distance(p, q[j].c), r2, nrads);
+                if (k >= 0 && q[j].row)
+                    q[j].row[k]++;
+            }
         }
     }
 }
@@ -200,16 +227,32 @@ typedef struct {
     int tid, nthreads;
 } oracle_job;
 
+/* Halos per group of queries (each halo makes 1 to 8 queries). */
+#define ORACLE_GROUP 64
+
 static void *oracle_worker(void *arg)
 {
     oracle_job *j = arg;
-    uint64_t h;
-    for (h = (uint64_t)j->tid; h < j->nh; h += (uint64_t)j->nthreads) {
-        float c[3];
-        memcpy(c, j->halos + h * j->stride, sizeof(c));
-        count_one_halo(j->particles, j->np, c, j->r2, j->nrads, j->rbox,
-                       j->counts ? &j->counts[h * (uint64_t)j->nrads] : NULL,
-                       &j->inbox);
+    oracle_query q[8 * ORACLE_GROUP];
+    uint64_t h = (uint64_t)j->tid;
+
+    while (h < j->nh) {
+        int nq = 0, g, img, d;
+        for (g = 0; g < ORACLE_GROUP && h < j->nh; g++, h += (uint64_t)j->nthreads) {
+            float c[3], centres[8][3];
+            int nimg;
+            memcpy(c, j->halos + h * j->stride, sizeof(c));
+            nimg = oracle_images(c, j->rbox, centres);
+            for (img = 0; img < nimg; img++, nq++) {
+                for (d = 0; d < 3; d++) {
+                    q[nq].c[d] = centres[img][d];
+                    q[nq].lo[d] = centres[img][d] - j->rbox;
+                    q[nq].hi[d] = centres[img][d] + j->rbox;
+                }
+                q[nq].row = j->counts ? &j->counts[h * (uint64_t)j->nrads] : NULL;
+            }
+        }
+        count_queries(j->particles, j->np, q, nq, j->r2, j->nrads, &j->inbox);
     }
     return NULL;
 }

@@ -42,6 +42,7 @@ EXPORTED = (
     "sjcs_reset_counts", "sjcs_add_particles", "sjcs_add_particles_device",
     "sjcs_sync", "sjcs_get_counts", "sjcs_last_chunk_pair_tests",
     "sjcs_set_option", "sjcs_get_option", "sjcs_autotune", "sjcs_last_chunk_timings",
+    "sjcs_phase_timings",
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
@@ -168,6 +169,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_last_chunk_pair_tests": (i32, [vp, ctypes.POINTER(u64)]),
         "sjcs_set_option": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
         "sjcs_last_chunk_timings": (i32, [vp, ctypes.POINTER(ctypes.c_float)]),
+        "sjcs_phase_timings": (i32, [vp, ctypes.POINTER(ctypes.c_float)]),
         "sjcs_get_option": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
         "sjcs_server_create": (i32, [vp, i32, u64, pp]),
         "sjcs_server_submit": (i32, [vp, vp, sz, u64, vp, i32, vp, ctypes.POINTER(u64)]),
@@ -332,6 +334,13 @@ class Engine:
         ms = (ctypes.c_float * 3)()
         self._check(self._lib.sjcs_last_chunk_timings(self._h, ms))
         return tuple(ms)
+
+    def phase_timings(self) -> dict:
+        """Device milliseconds: the last chunk's upload, index build and
+        count, the last halo preparation and the last read-out."""
+        ms = (ctypes.c_float * 5)()
+        self._check(self._lib.sjcs_phase_timings(self._h, ms))
+        return dict(zip(("upload", "index_build", "count", "halo_sort", "readout"), ms))
 
     def device_array(self, host: np.ndarray) -> int:
         """Copies a host array into new device memory; returns the device

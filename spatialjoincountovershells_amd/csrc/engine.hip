@@ -231,6 +231,11 @@ struct sjcs_engine {
   hipEvent_t built[2] = {nullptr, nullptr};     // slot's sort done
   hipEvent_t counted[2] = {nullptr, nullptr};   // slot's count done
   hipEvent_t uploaded[2] = {nullptr, nullptr};  // slot's pinned buffer read
+  // every read of the slot's device staging (cin / cuv) queued so far is
+  // done: the pieces of one chunk alternate between slots but all read the
+  // staging of the slot the chunk was uploaded to, and in graph mode their
+  // sorts run on `stream`, not in upload order on `bstream`
+  hipEvent_t stage_read[2] = {nullptr, nullptr};
   DevBuf in;  // halo / query staging on the device
   DevBuf ws;  // sort / scan workspace
   DevBuf inbox;
@@ -265,6 +270,11 @@ struct sjcs_engine {
   // last chunk: upload start/end, sort start/end, count start/end
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
+  // last halo preparation (keys, Morton sort, tiling) and last read-out
+  // of the counts into input order: start/end each
+  hipEvent_t hev[2] = {nullptr, nullptr};
+  hipEvent_t rev[2] = {nullptr, nullptr};
+  bool halo_timed = false, readout_timed = false;
 };
 
 struct sjcs_index {
@@ -407,7 +417,7 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
 
 // Halo centres from a device AoS (packed xyz) -> Morton-sorted SoA + rows.
 int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride, uint64_t n,
-               bool tiles) {
+               bool tiles, bool timed) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 halos");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
   DevBuf key[2], x[2], y[2], z[2], row[2];
@@ -421,6 +431,7 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
     if (h == hipSuccess) h = ensure(row[i], b);
   }
   if (h == hipSuccess) rc = ensure_ws(e, radix_workspace_words(n));
+  if (h == hipSuccess && rc == SJCS_OK && timed) h = hipEventRecord(e->hev[0], e->stream);
   if (h == hipSuccess && rc == SJCS_OK)
     h = halo_keys(d_xyz, stride, n, key[0].as<uint32_t>(), x[0].as<float>(), y[0].as<float>(),
                   z[0].as<float>(), row[0].as<uint32_t>(), e->stream);
@@ -492,6 +503,11 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
       if (h != hipSuccess) rc = hip_fail(e, h, "halo tiling");
       sel = o;
     }
+  }
+  if (rc == SJCS_OK && timed) {
+    h = hipEventRecord(e->hev[1], e->stream);
+    if (h != hipSuccess) rc = hip_fail(e, h, "hipEventRecord");
+    e->halo_timed = rc == SJCS_OK;
   }
   if (rc == SJCS_OK) {
     hs.release();
@@ -748,6 +764,23 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint
 
 int piece_done(sjcs_engine* e, uint64_t n);
 
+// Records stage_read[s] on stream st when p points into slot s's device
+// staging (cin or cuv): work queued on st so far is the last reader of it.
+int note_staging_read(sjcs_engine* e, const void* p, hipStream_t st) {
+  const char* c = static_cast<const char*>(p);
+  for (int s = 0; s < 2; ++s) {
+    const DevBuf* bufs[2] = {&e->cin[s], &e->cuv[s]};
+    for (const DevBuf* b : bufs) {
+      const char* lo = b->as<char>();
+      if (lo && c >= lo && c < lo + b->bytes) {
+        HIPCALL(e, hipEventRecord(e->stage_read[s], st), "hipEventRecord");
+        return SJCS_OK;
+      }
+    }
+  }
+  return SJCS_OK;
+}
+
 // Graph mode: sort and count of a piece on `stream` as one captured graph,
 // captured on first use and replayed while nothing it holds has changed.
 // The sort waits for the slot's upload; it follows the previous count in
@@ -797,6 +830,8 @@ int graph_piece(sjcs_engine* e, const ParticleSource& src, uint64_t n, const Gri
     cg.epoch = e->epoch;
   }
   HIPCALL(e, hipGraphLaunch(cg.exec, e->stream), "hipGraphLaunch");
+  rc = note_staging_read(e, in, e->stream);
+  if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->built[sl], e->stream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->counted[sl], e->stream), "hipEventRecord");
@@ -818,12 +853,17 @@ int add_piece(sjcs_engine* e, ParticleSource src, uint64_t n) {
     // side, need the positions first: decode into the slot's staging once
     // the sort that last read it is done
     const int sl = e->slot;
-    if (e->cin[sl].bytes < 12 * (size_t)n)
+    if (e->cin[sl].bytes < 12 * (size_t)n) {
       HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+      HIPCALL(e, hipEventSynchronize(e->stage_read[sl]), "hipEventSynchronize");
+    }
     HIPCALL(e, ensure(e->cin[sl], 12 * (size_t)n), "hipMalloc(points)");
     HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+    HIPCALL(e, hipStreamWaitEvent(e->bstream, e->stage_read[sl], 0), "hipStreamWaitEvent");
     HIPCALL(e, decode_gadget(src.uv, n, src.gd, e->cin[sl].as<float>(), e->bstream),
             "decode compressed particles");
+    int rn = note_staging_read(e, src.uv, e->bstream);
+    if (rn != SJCS_OK) return rn;
     if (e->graph) HIPCALL(e, hipEventRecord(e->uploaded[sl], e->bstream), "hipEventRecord");
     src = ParticleSource{};
     src.xyz = e->cin[sl].as<float>();
@@ -837,6 +877,10 @@ int add_piece(sjcs_engine* e, ParticleSource src, uint64_t n) {
     HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[e->last], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[2], e->bstream), "hipEventRecord");
   int rc = build_index(e, e->ibs[sl], src, n, g, e->bstream);
+  if (rc == SJCS_OK)
+    rc = note_staging_read(e, src.uv ? static_cast<const void*>(src.uv)
+                                     : static_cast<const void*>(src.xyz),
+                           e->bstream);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipEventRecord(e->ev[3], e->bstream), "hipEventRecord");
   HIPCALL(e, hipEventRecord(e->built[sl], e->bstream), "hipEventRecord");
@@ -908,10 +952,14 @@ int stage_chunk(sjcs_engine* e, const void* src, size_t bytes, DevBuf& dst) {
     HIPCALL(e, hipHostMalloc(&e->cpin[sl], pin, hipHostMallocDefault), "hipHostMalloc");
     e->cpin_bytes[sl] = pin;
   }
-  if (dst.bytes < bytes) HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+  if (dst.bytes < bytes) {
+    HIPCALL(e, hipEventSynchronize(e->counted[sl]), "hipEventSynchronize");
+    HIPCALL(e, hipEventSynchronize(e->stage_read[sl]), "hipEventSynchronize");
+  }
   HIPCALL(e, ensure(dst, bytes), "hipMalloc(points)");
   if (bytes) memcpy(e->cpin[sl], src, bytes);
   HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
+  HIPCALL(e, hipStreamWaitEvent(e->bstream, e->stage_read[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[0], e->bstream), "hipEventRecord");
   if (bytes)
     HIPCALL(e, hipMemcpyAsync(dst.p, e->cpin[sl], bytes, hipMemcpyHostToDevice, e->bstream),
@@ -943,13 +991,19 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
   if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->bstream, hipStreamNonBlocking);
   for (int i = 0; i < 6 && h == hipSuccess; ++i) h = hipEventCreate(&e->ev[i]);
   for (int i = 0; i < 2 && h == hipSuccess; ++i) {
+    h = hipEventCreate(&e->hev[i]);
+    if (h == hipSuccess) h = hipEventCreate(&e->rev[i]);
+  }
+  for (int i = 0; i < 2 && h == hipSuccess; ++i) {
     h = hipEventCreateWithFlags(&e->built[i], hipEventDisableTiming);
     if (h == hipSuccess) h = hipEventCreateWithFlags(&e->counted[i], hipEventDisableTiming);
     if (h == hipSuccess) h = hipEventCreateWithFlags(&e->uploaded[i], hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->stage_read[i], hipEventDisableTiming);
     // recorded once so that the first waits on them return at once
     if (h == hipSuccess) h = hipEventRecord(e->built[i], e->stream);
     if (h == hipSuccess) h = hipEventRecord(e->counted[i], e->stream);
     if (h == hipSuccess) h = hipEventRecord(e->uploaded[i], e->stream);
+    if (h == hipSuccess) h = hipEventRecord(e->stage_read[i], e->stream);
   }
   // the pair-test total (u64) and the halo sort's run flag (u32)
   if (h == hipSuccess) h = ensure(e->inbox, 16);
@@ -983,6 +1037,7 @@ void sjcs_engine_destroy(sjcs_engine* e) {
     if (e->built[i]) (void)hipEventDestroy(e->built[i]);
     if (e->counted[i]) (void)hipEventDestroy(e->counted[i]);
     if (e->uploaded[i]) (void)hipEventDestroy(e->uploaded[i]);
+    if (e->stage_read[i]) (void)hipEventDestroy(e->stage_read[i]);
   }
   e->in.release();
   e->ws.release();
@@ -990,6 +1045,10 @@ void sjcs_engine_destroy(sjcs_engine* e) {
   if (e->pinned) (void)hipHostFree(e->pinned);
   for (int i = 0; i < 6; ++i)
     if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  for (int i = 0; i < 2; ++i) {
+    if (e->hev[i]) (void)hipEventDestroy(e->hev[i]);
+    if (e->rev[i]) (void)hipEventDestroy(e->rev[i]);
+  }
   if (e->bstream) (void)hipStreamDestroy(e->bstream);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -1026,7 +1085,7 @@ int sjcs_set_halos(sjcs_engine* e, const float* points, size_t stride_bytes, uin
   HIPCALL(e, ensure(e->in, bytes), "hipMalloc(halos)");
   if (nhalos)
     HIPCALL(e, hipMemcpy(e->in.p, points, bytes, hipMemcpyHostToDevice), "hipMemcpy(halos)");
-  rc = sort_halos(e, e->halos, e->in.as<float>(), stride_bytes, nhalos, true);
+  rc = sort_halos(e, e->halos, e->in.as<float>(), stride_bytes, nhalos, true, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -1039,7 +1098,7 @@ int sjcs_set_halos_device(sjcs_engine* e, const float* d_xyz, uint64_t nhalos) {
   if (rc == SJCS_OK) rc = quiesce(e);  // in-flight counts read the halos
   if (rc != SJCS_OK) return rc;
   ++e->epoch;
-  rc = sort_halos(e, e->halos, d_xyz, 12, nhalos, true);
+  rc = sort_halos(e, e->halos, d_xyz, 12, nhalos, true, true);
   if (rc != SJCS_OK) return rc;
   e->have_halos = true;
   return prepare_counts(e);
@@ -1159,11 +1218,14 @@ int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint6
   if (e->readout_gen != e->counts_gen) {
     const size_t all = (size_t)e->halos.n * K;
     HIPCALL(e, ensure(e->ucounts, all * sizeof(uint64_t)), "hipMalloc(counts)");
+    HIPCALL(e, hipEventRecord(e->rev[0], e->stream), "hipEventRecord");
     HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
                               e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
                               e->halos.row.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
                               e->shells.nrads, e->stream),
             "read out counts");
+    HIPCALL(e, hipEventRecord(e->rev[1], e->stream), "hipEventRecord");
+    e->readout_timed = true;
     e->readout_gen = e->counts_gen;
   }
   HIPCALL(e, hipMemcpyAsync(counts, e->ucounts.as<uint64_t>() + first * K, n * sizeof(uint64_t),
@@ -1378,6 +1440,21 @@ int sjcs_last_chunk_timings(sjcs_engine* e, float ms[3]) {
   return SJCS_OK;
 }
 
+int sjcs_phase_timings(sjcs_engine* e, float ms[5]) {
+  if (!e || !ms) return SJCS_ERR_INVALID;
+  int rc = sjcs_last_chunk_timings(e, ms);
+  if (rc != SJCS_OK) return rc;
+  ms[3] = ms[4] = 0.0f;
+  rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);
+  if (rc != SJCS_OK) return rc;
+  if (e->halo_timed)
+    HIPCALL(e, hipEventElapsedTime(&ms[3], e->hev[0], e->hev[1]), "hipEventElapsedTime");
+  if (e->readout_timed)
+    HIPCALL(e, hipEventElapsedTime(&ms[4], e->rev[0], e->rev[1]), "hipEventElapsedTime");
+  return SJCS_OK;
+}
+
 int sjcs_device_alloc(sjcs_engine* e, size_t bytes, void** d_ptr) {
   if (!e || !d_ptr) return SJCS_ERR_INVALID;
   *d_ptr = nullptr;
@@ -1437,7 +1514,8 @@ int sjcs_index_create(sjcs_engine* e, uint64_t ndata, sjcs_index** out) {
   *out = nullptr;
   int rc = use_device(e);
   if (rc != SJCS_OK) return rc;
-  if (ndata > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 particles");
+  // the count kernels address particles with 32-bit byte offsets
+  if (ndata > 0x3fffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^30-1 particles");
   sjcs_index* ix = new sjcs_index();
   ix->e = e;
   ix->cap = ndata;
@@ -1502,7 +1580,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   if (rc == SJCS_OK && !(sh.view.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
     rc = fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
   if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
-  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), 12, n, false);
+  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), 12, n, false, false);
   const size_t nc = (size_t)n * (size_t)nrads;
   std::vector<uint64_t> host;
   if (rc == SJCS_OK) {
@@ -1924,13 +2002,20 @@ int sjcs_server_submit(sjcs_server* sv, const float* centrals, size_t stride_byt
     return SJCS_ERR_INVALID;
   }
   auto r = std::make_shared<ServerRequest>();
+  shell_table(radiuses_sq, nrads, &r->shells, &r->lut);
+  // a request's counts are u32 on the device: a particle counts once per
+  // image it falls in, 8 when the box is about as wide as the periodic box
+  if (!(r->shells.rbox < 499.0f) && sv->ix->n > 0xffffffffull / 8) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->err = "submit: index too large for this radius (counts could pass 2^32)";
+    return SJCS_ERR_INVALID;
+  }
   r->n = n;
   r->nrads = nrads;
   r->xyz.resize(3 * (size_t)n);
   const char* src = reinterpret_cast<const char*>(centrals);
   for (uint64_t i = 0; i < n; ++i) memcpy(&r->xyz[3 * i], src + i * stride_bytes, 12);
   r->r2.assign(radiuses_sq, radiuses_sq + nrads);
-  shell_table(radiuses_sq, nrads, &r->shells, &r->lut);
   r->out = counts;
   r->need = request_bytes(n, nrads);
   std::lock_guard<std::mutex> lk(sv->mu);

@@ -316,3 +316,30 @@ def parse_result(text: bytes, k: int):
         tlab.append(head)
     return (np.array(ids, np.uint64), np.array(rows, np.uint64).reshape(-1, k),
             np.array(total, np.uint64), tlab)
+
+
+def format_result(ids, counts, r2):
+    """The result text as outputResult writes it (src/astr_rcount/main.c:
+    371-442), restated in plain Python for the writer tests: labels from
+    the FP32 sqrt of each squared threshold printed with %f, "id %lu" and
+    one line per shell, then the Total block, whose running sums live in
+    halo 0's row (total = &counts[0], main.c:400): halo 0's counts are
+    printed first and then added to themselves, so Total = 2 x halo 0 +
+    the other halos, in u64 arithmetic."""
+    k = len(r2)
+    rad = [float(np.float32(np.sqrt(np.float64(v)))) for v in r2]
+    labels = ["  [0.0 ~ %f]: " % rad[0]] + ["  [%f ~ %f]: " % (rad[i - 1], rad[i])
+                                          for i in range(1, k)]
+    counts = np.asarray(counts, np.uint64).reshape(-1, k)
+    out = []
+    for hid, row in zip(np.asarray(ids, np.uint64).tolist(), counts.tolist()):
+        out.append("id %d\n" % hid)
+        out.extend(lab + "%d\n" % v for lab, v in zip(labels, row))
+    if counts.shape[0]:
+        with np.errstate(over="ignore"):
+            total = counts[0] + counts.sum(axis=0, dtype=np.uint64)
+    else:
+        total = np.zeros(k, np.uint64)
+    out.append("\nTotal\n")
+    out.extend(lab + "%d\n" % v for lab, v in zip(labels, total.tolist()))
+    return "".join(out).encode()

@@ -59,6 +59,26 @@ def oracle_chunks(chunks, halos, r2):
     return tot, w
 
 
+def face_weighted_sample(rng, h, n, rbox, frac=0.3):
+    """n distinct halo indices, at least `frac` of them with a coordinate
+    within rbox of a face (so their boxes leave the periodic box and they
+    have images): every corner halo up to a tenth of those, edge halos (two
+    axes near a face) up to a third, the rest face halos; the others drawn
+    uniformly."""
+    d = np.minimum(h, np.float32(1000.0) - h)
+    nax = (d < rbox).sum(axis=1)
+    k = int(np.ceil(frac * n))
+    corner = np.nonzero(nax == 3)[0][: max(1, k // 10)]
+    edge = np.nonzero(nax == 2)[0]
+    edge = rng.choice(edge, min(edge.size, k // 3), replace=False)
+    face = np.nonzero(nax == 1)[0]
+    face = rng.choice(face, k - corner.size - edge.size, replace=False)
+    near = np.concatenate([corner, edge, face])
+    rest = np.setdiff1d(np.arange(h.shape[0]), near)
+    rest = rng.choice(rest, n - near.size, replace=False)
+    return np.concatenate([near, rest]), near.size
+
+
 def assert_counts_equal(got, want, halos):
     if not np.array_equal(got, want):
         rows = np.nonzero((got != want).any(axis=1))[0]
@@ -99,6 +119,54 @@ def test_cli_golden_case(built, case, tmp_path):
     _, rids, _ = t.case_inputs(case, filter_as_reference_runs=True)
     eids, *_ = t.parse_result(t.read_expected_text(case["expected"]), k)
     assert np.array_equal(eids, rids)
+
+
+def write_halo_list(path, ids, pts):
+    """The reference's halo text (src/Gadget/BufHalo.c:76-119): 16 header
+    lines, then id and 7 more columns with x, y, z in columns 9-11; %.9g
+    gives back the same FP32 values through strtof."""
+    with open(path, "w") as f:
+        f.writelines(f"# header {i}\n" for i in range(16))
+        f.writelines(f"{i} 1.5 0 0 0 0 0 0 {x:.9g} {y:.9g} {z:.9g} 0\n"
+                     for i, (x, y, z) in zip(ids.tolist(), pts.tolist()))
+
+
+@pytest.mark.parametrize("threads", [1, 16])
+def test_cli_result_file_many_batches(built, tmp_path, threads):
+    """The drop-in CLI end to end on 140000 halos (five batches of the
+    result writer, the counts fetched from the device batch by batch while
+    the previous one is formatted) and two particle files: its result file
+    is byte for byte the plain formatter's text of the oracle's counts."""
+    p, h = gen_uniform(51, 400_000, 140_000)
+    ids = np.arange(140_000, dtype=np.uint64) * np.uint64(7919) + np.uint64(3)
+    files = [tmp_path / "p0.dat", tmp_path / "p1.dat"]
+    p[:250_000].tofile(files[0])
+    p[250_000:].tofile(files[1])
+    hf = tmp_path / "halos.list"
+    write_halo_list(hf, ids, h)
+    out = tmp_path / "result.txt"
+    args = [built.CLI_PATH, "--thread", str(threads), "--next-setup", "2", "--particle-files",
+            str(files[0]), str(files[1]), "--halo-file", str(hf), "--radius", "0.001:5:8",
+            "--output-file", str(out)]
+    print("cli many batches, threads", threads, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 8)
+    want, _ = oracle_chunks([p[:250_000], p[250_000:]], h, r2)
+    got = out.read_bytes()
+    expect = t.format_result(ids, want, r2)
+    assert len(got) == len(expect) and got == expect
+
+
+def test_dpps_equivalence_on_the_gpu_host():
+    """The DPPS probe (the FP32 distance as the reference's SSE4.1 dot
+    product rounds it, against the oracle's plain expression) on the host
+    of the GPU box: the CPU whose reference binary the benchmark times."""
+    t.oracle()
+    out = subprocess.run([t.DPPS_PROBE, "3000000", "4242"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("dpps ok")
 
 
 @pytest.mark.parametrize("missing", ["particles", "halos", "second_particles", "compressed"])
@@ -567,6 +635,15 @@ def test_server_alone_equals_mix(built, eng):
             if h.shape[0] <= 700:
                 want, _ = t.oracle_count(p, h, r2, threads=8)
                 assert_counts_equal(a, want, h)
+            else:
+                # requests of 4096 halos or more are Morton-ordered and read
+                # out through the permutation: equal to the index API (whose
+                # own test checks it against the oracle), and a sample of
+                # rows against the oracle directly
+                assert_counts_equal(a, ix.multi_range_count(h, r2), h)
+                rows = np.random.default_rng(h.shape[0]).choice(h.shape[0], 300, replace=False)
+                want, _ = t.oracle_count(p, h[rows], r2, threads=8)
+                assert_counts_equal(a[rows], want, h[rows])
     finally:
         sv.close()
         ix.close()
@@ -852,11 +929,18 @@ def test_pieces_equal_whole_chunks(built, eng, graph):
     rng = np.random.default_rng(4242)
     _, r2 = t.oracle_thresholds(0.001, 4.0, 12)
     p, h = gen_uniform(43, 250_000, 3_000)
+    q, _ = gen_uniform(44, 260_000, 1)
     h = np.vstack([h, p[:1500] + np.float32(0.5)]).astype(np.float32)
     one = gadget_params(built, 100 ** 3, 1000.0, -4.0, 4.0, 16, 1)
     many = gadget_params(built, 300 ** 3, 900.0, -3.0, 1.0, 12, 1, 3)
     uv1 = rng.integers(0, 1 << 16, (300_000, 3), dtype=np.uint16)
     uv3 = rng.integers(0, 1 << 12, (300_000, 3), dtype=np.uint16)
+    # with 70001 particles per piece: 250k, 260k and 280k particles are 4
+    # pieces each -- an even number, so the next chunk is uploaded into the
+    # staging that the previous chunk's last piece (run from the other slot)
+    # may still be reading; 300k are 5 pieces
+    uv1b = rng.integers(0, 1 << 16, (280_000, 3), dtype=np.uint16)
+    uv3b = rng.integers(0, 1 << 12, (280_000, 3), dtype=np.uint16)
     eng.set_shells(r2)
     eng.set_halos(h)
     res = []
@@ -868,8 +952,13 @@ def test_pieces_equal_whole_chunks(built, eng, graph):
             assert eng.get_option("piece_max") == cap
             eng.reset_counts()
             eng.add_particles(p)
+            eng.add_particles(q)
+            eng.add_compressed_particles(uv1b, one)
             eng.add_compressed_particles(uv1, one)
+            eng.add_compressed_particles(uv3b, many)
+            eng.add_compressed_particles(uv3b[::-1], many)
             eng.add_compressed_particles(uv3, many)
+            eng.add_particles(q[::-1])
             res.append(eng.counts())
     finally:
         eng.set_option("piece_max", 0)
@@ -939,9 +1028,11 @@ def test_full_size_config2_properties(eng):
         eng.set_option("cell_size_milli", 0)
         eng.set_option("index_sort", 0)
         eng.device_free(d)
-    sample = rng.choice(h.shape[0], 12, replace=False)
+    sample, nnear = face_weighted_sample(rng, h, 1000, 5.0)
+    print(f"full size: oracle on {sample.size} halos ({nnear} within r_box of a face)",
+          flush=True)
     want, _ = t.oracle_count(p, h[sample], r2, threads=16)
-    assert np.array_equal(base[sample], want)
+    assert_counts_equal(base[sample], want, h[sample])
 
 
 def test_full_size_config3_properties(eng):
@@ -982,7 +1073,11 @@ def test_full_size_config3_properties(eng):
         eng.set_option("graph", 0)
         for d in ds:
             eng.device_free(d)
-    sample = rng.choice(h.shape[0], 8, replace=False)
-    want = sum(t.oracle_count(p, h[sample], r2, threads=16)[0] for p in files)
-    print("config 3: oracle sample done", flush=True)
-    assert np.array_equal(total[sample], want)
+    sample, nnear = face_weighted_sample(rng, h, 500, 5.0)
+    print(f"config 3: oracle on {sample.size} halos ({nnear} within r_box of a face)",
+          flush=True)
+    want = np.zeros((sample.size, 40), np.uint64)
+    for i, p in enumerate(files):
+        want += t.oracle_count(p, h[sample], r2, threads=16)[0]
+        print(f"config 3: oracle file {i} done", flush=True)
+    assert_counts_equal(total[sample], want, h[sample])

@@ -164,6 +164,54 @@ def test_result_writer_byte_identical(host, case, tmp_path):
         assert out.read_bytes() == t.read_expected_text(case["expected"])
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_plain_formatter_equals_reference_files(case):
+    """The plain formatter the large writer tests compare with gives every
+    golden result file (the reference's own output) byte for byte from the
+    oracle's counts: labels, %lu and the Total block."""
+    if case.get("filter_file"):
+        pytest.skip("the reference's filter reads an uninitialised count; see test_cli")
+    mn, mx, k = t.parse_radius(case["radius"])
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    chunks, ids, pts = t.case_inputs(case)
+    tot = np.zeros((pts.shape[0], k), np.uint64)
+    for ch in chunks:
+        tot += t.oracle_count(ch, pts, r2)[0]
+    assert t.format_result(ids, tot, r2) == t.read_expected_text(case["expected"])
+
+
+@pytest.mark.parametrize("spec,nthreads", [("0.001:5:40", 1), ("0.001:5:40", 16),
+                                           ("0.01:20:64", 16), ("2:3:1", 7)])
+def test_result_writer_many_batches(host, tmp_path, spec, nthreads):
+    """140000 halos -- five of the writer's 32768-row batches, the last one
+    partial -- with ids and counts over the whole u64 range (sums that wrap
+    in the Total block included), written by the host writer with 1 to 16
+    threads: byte for byte the plain formatter's text."""
+    rng = np.random.default_rng(len(spec) * 31 + nthreads)
+    mn, mx, k = t.parse_radius(spec)
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    n = 140_000
+    ids = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    ids[:3] = [0, 1, np.iinfo(np.uint64).max]
+    counts = rng.integers(0, 1 << 20, (n, k), dtype=np.uint64)
+    counts[rng.random((n, k)) < 0.001] = np.iinfo(np.uint64).max - np.uint64(5)
+    counts[0, 0] = np.iinfo(np.uint64).max  # halo 0 is doubled in the Total block
+    arr = (Halo * n)()
+    ctypes.memmove(arr, np.rec.fromarrays([ids, np.zeros(n, np.float32)] +
+                                          [np.zeros(n, np.float32)] * 3,
+                                          dtype=[("id", "<u8"), ("m", "<f4"), ("x", "<f4"),
+                                                 ("y", "<f4"), ("z", "<f4")]).tobytes(),
+                   ctypes.sizeof(Halo) * n)
+    hs = Halos(ctypes.cast(arr, ctypes.POINTER(Halo)), n)
+    out = tmp_path / "out.txt"
+    rc = host.sjcs_write_result(str(out).encode(), ctypes.byref(hs), counts.ctypes.data,
+                                r2.ctypes.data, k, nthreads)
+    assert rc == 0
+    got = out.read_bytes()
+    want = t.format_result(ids, counts, r2)
+    assert got == want
+
+
 def test_particle_count_ignores_trailing_bytes(host, tmp_path):
     p = tmp_path / "p.dat"
     p.write_bytes(np.arange(30, dtype=np.float32).tobytes() + b"\x01\x02\x03\x04\x05")
