@@ -337,8 +337,9 @@ int ensure_ws(sjcs_engine* e, uint64_t words) {
 }
 
 // Shell view (device pointers left null) and first-match table of nrads
-// squared thresholds; host only.
-void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry>* out_lut) {
+// squared thresholds, host only: the table's entries go to lut[0, *nlut),
+// at most kLutSize of them (one entry when the thresholds are not sorted).
+void shell_table_into(const float* r2, int nrads, Shells* out, ShellEntry* lut, int* nlut_out) {
   bool sorted = true;
   for (int k = 0; k < nrads && sorted; ++k) {
     uint32_t b = float_bits(r2[k]);
@@ -352,7 +353,8 @@ void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry
   v.r2max_hi = r2[nrads - 1] * (1.0f + 1e-5f);
   // src/objects/ArraySTR/MultiRangeCount.c:19: float radius = sqrt(r2[K-1])
   v.rbox = (float)sqrt((double)r2[nrads - 1]);
-  std::vector<ShellEntry> lut(1, ShellEntry{0u, 0.0f});
+  lut[0] = ShellEntry{0u, 0.0f};
+  *nlut_out = 1;
   if (sorted) {
     // prefer a table based at bits 0 (one shift finds the bucket); fall back
     // to one based at r2[0] when buckets that coarse hold two thresholds
@@ -363,7 +365,6 @@ void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry
       int shift = 0;
       while ((range >> shift) >= (uint32_t)kLutSize) ++shift;
       const int nlut = (int)(range >> shift) + 1;
-      std::vector<ShellEntry> t(nlut, ShellEntry{0u, 0.0f});
       bool one_step = true;
       for (int b = 0; b < nlut; ++b) {
         // smallest and largest distance of bucket b (bucket 0 also takes
@@ -374,22 +375,29 @@ void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry
         const float dmax = bits_float(hib);
         int k = 0;
         while (k < nrads - 1 && !(dmin < r2[k])) ++k;
-        t[b].off = 4u * (uint32_t)k;
-        t[b].t = r2[k];
+        lut[b].off = 4u * (uint32_t)k;
+        lut[b].t = r2[k];
         // k + 1 is the answer for any d >= r2[k] of this bucket only if the
         // next threshold lies above the bucket
         if (k + 1 < nrads && !(dmax < r2[k + 1])) one_step = false;
       }
       if (!one_step && attempt == 0) continue;
-      lut.swap(t);
       v.lut_base = base;
       v.lut_shift = shift;
       v.nlut = nlut;
       v.one_step = one_step ? 1 : 0;
+      *nlut_out = nlut;
       break;
     }
   }
   *out = v;
+}
+
+void shell_table(const float* r2, int nrads, Shells* out, std::vector<ShellEntry>* out_lut) {
+  std::vector<ShellEntry> lut(kLutSize);
+  int nlut = 1;
+  shell_table_into(r2, nrads, out, lut.data(), &nlut);
+  lut.resize(nlut);
   out_lut->swap(lut);
 }
 
@@ -670,13 +678,14 @@ int prepare_counts(sjcs_engine* e) {
 // halo per wavefront kernel, else "lanes_per_halo" (0 = automatic).
 // Automatic: 64 lanes when a box holds thousands of particles (the row walk
 // then dominates), else 16.
-int lanes(const sjcs_engine* e, uint64_t n) {
+int lanes(const sjcs_engine* e, uint64_t n, float rbox) {
   if (e->kernel == 1) return 64;
   if (e->lanes_per_halo) return (int)e->lanes_per_halo;
-  const double side = 2.0 * (double)e->shells.view.rbox / 1000.0;
+  const double side = 2.0 * (double)rbox / 1000.0;
   const double per_box = (double)n * side * side * side;
   return !(per_box <= 4096.0) ? 64 : 16;
 }
+int lanes(const sjcs_engine* e, uint64_t n) { return lanes(e, n, e->shells.view.rbox); }
 int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
 
 // Most images of one halo a particle can lie in: 1 unless the query box is
@@ -1281,8 +1290,8 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
   rc = quiesce(e);
   if (rc != SJCS_OK) return rc;
   const size_t nc = (size_t)e->halos.n * (size_t)e->shells.nrads;
-  DevBuf scratch;
-  HIPCALL(e, ensure(scratch, 4 * (nc ? nc : 1)), "hipMalloc(autotune)");
+  DevBuf trial_counts;
+  HIPCALL(e, ensure(trial_counts, 4 * (nc ? nc : 1)), "hipMalloc(autotune)");
   struct Setting {
     int64_t cell, xs, lanes, seg, tiles;
   };
@@ -1306,7 +1315,7 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
       HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
       r = build_index(e, e->ibs[0], packed(d_xyz), n, g, e->stream);
       if (r != SJCS_OK) return r;
-      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, scratch.as<uint32_t>(), false, nullptr,
+      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, trial_counts.as<uint32_t>(), false, nullptr,
                              e->stream),
               "autotune count");
       HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
@@ -1352,7 +1361,7 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
   ++e->epoch;
   e->last = -1;
   e->timed = false;
-  scratch.release();
+  trial_counts.release();
   if (rc == SJCS_OK && best_ms) *best_ms = best;
   return rc;
 }
@@ -1588,7 +1597,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
     if (h == hipSuccess) h = ensure(ucounts, nc * sizeof(uint64_t));
     if (h == hipSuccess)
       h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint32_t>(), false,
-                      nullptr, lanes(e, ix->n), seg_len(e), e->stream);
+                      nullptr, lanes(e, ix->n, sh.view.rbox), seg_len(e), e->stream);
     if (h == hipSuccess)
       h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.row.as<uint32_t>(),
                          ucounts.as<uint64_t>(), n, nrads, e->stream);

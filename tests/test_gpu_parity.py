@@ -145,7 +145,7 @@ def test_cli_result_file_many_batches(built, tmp_path, threads):
     hf = tmp_path / "halos.list"
     write_halo_list(hf, ids, h)
     out = tmp_path / "result.txt"
-    args = [built.CLI_PATH, "--thread", str(threads), "--next-setup", "2", "--particle-files",
+    args = [built.CLI_PATH, "--thread", str(threads), "--next-setup", "1", "--particle-files",
             str(files[0]), str(files[1]), "--halo-file", str(hf), "--radius", "0.001:5:8",
             "--output-file", str(out)]
     print("cli many batches, threads", threads, flush=True)
@@ -929,7 +929,7 @@ def test_pieces_equal_whole_chunks(built, eng, graph):
     rng = np.random.default_rng(4242)
     _, r2 = t.oracle_thresholds(0.001, 4.0, 12)
     p, h = gen_uniform(43, 250_000, 3_000)
-    q, _ = gen_uniform(44, 260_000, 1)
+    q, _ = gen_uniform(44, 260_000, 3)
     h = np.vstack([h, p[:1500] + np.float32(0.5)]).astype(np.float32)
     one = gadget_params(built, 100 ** 3, 1000.0, -4.0, 4.0, 16, 1)
     many = gadget_params(built, 300 ** 3, 900.0, -3.0, 1.0, 12, 1, 3)
