@@ -300,6 +300,19 @@ int sjcs_index_count(sjcs_index *ix, const float *centrals, size_t stride_bytes,
                      uint64_t *counts);
 void sjcs_index_destroy(sjcs_index *ix);
 
+/* sjcs_index_count without any device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d_centrals: n packed FP32 xyz on the device; d_counts:
+ * u64 [n][nrads] on the device, counts[h*nrads + k] += shell-k count;
+ * radiuses_sq: host, at most 1024 shells.  Its working memory is a fixed
+ * 32 MiB array in the library, one per device: n * (20 + 4*nrads) bytes
+ * plus 12 KiB must fit (SJCS_ERR_INVALID otherwise).  Every call on a
+ * device shares that array: calls that overlap on one device (two host
+ * threads, two engines) race.  Returns when the counts are added. */
+int sjcs_index_count_noalloc(sjcs_index *ix, const float *d_centrals, uint64_t n,
+                             const float *radiuses_sq, int nrads, uint64_t *d_counts);
+
 #ifdef __cplusplus
 }
 #endif

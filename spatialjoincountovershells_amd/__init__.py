@@ -44,7 +44,7 @@ EXPORTED = (
     "sjcs_set_option", "sjcs_get_option", "sjcs_autotune", "sjcs_last_chunk_timings",
     "sjcs_phase_timings",
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
-    "sjcs_index_count", "sjcs_index_destroy",
+    "sjcs_index_count", "sjcs_index_destroy", "sjcs_index_count_noalloc",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
     "sjcs_host_alloc", "sjcs_host_free", "sjcs_get_counts_range",
     "sjcs_gadget_decode_params", "sjcs_add_compressed_particles",
@@ -187,6 +187,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_index_build": (i32, [vp]),
         "sjcs_index_count": (i32, [vp, vp, sz, u64, vp, i32, vp]),
         "sjcs_index_destroy": (None, [vp]),
+        "sjcs_index_count_noalloc": (i32, [vp, vp, u64, vp, i32, vp]),
         "sjcs_device_alloc": (i32, [vp, sz, pp]),
         "sjcs_device_free": (i32, [vp, vp]),
         "sjcs_copy_to_device": (i32, [vp, vp, vp, sz]),
@@ -400,6 +401,14 @@ class ArraySTR:
         self._e._check(self._lib.sjcs_index_count(self._h, _ptr(c), 12, c.shape[0], _ptr(r2),
                                                   r2.size, _ptr(counts)))
         return counts
+
+    def multi_range_count_noalloc(self, d_centrals: int, n: int, radiuses_sq, d_counts: int):
+        """sjcs_index_count_noalloc: centres and u64 counts [n][K] already on
+        the device (counts +=); the library allocates nothing for it."""
+        r2 = _shells(radiuses_sq)
+        self._e._check(self._lib.sjcs_index_count_noalloc(
+            self._h, ctypes.c_void_p(d_centrals), n, _ptr(r2), r2.size,
+            ctypes.c_void_p(d_counts)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -193,8 +193,39 @@ float bits_float(uint32_t b) {
 
 }  // namespace
 
+// Device scratch of sjcs_index_count_noalloc, for callers that cannot
+// allocate device memory: a fixed array in the library's code object, one
+// copy on every device the library runs on, held until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy (on the host
+// it names a stand-in, not device memory): it uses the device address
+// hipGetSymbolAddress gives, looked up once per device (scratch_address).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t h = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (h != hipSuccess) return h;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 struct sjcs_engine {
   int device = 0;
+  void* scratch = nullptr;  // device address of `scratch` on this device
   hipStream_t stream = nullptr;   // counts, halo sorts, the index API
   hipStream_t bstream = nullptr;  // chunk uploads and cell sorts
   std::string err;
@@ -996,6 +1027,7 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
   sjcs_engine* e = new sjcs_engine();
   e->device = device;
   h = hipSetDevice(device);
+  if (h == hipSuccess) h = scratch_address(device, &e->scratch);
   if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->bstream, hipStreamNonBlocking);
   for (int i = 0; i < 6 && h == hipSuccess; ++i) h = hipEventCreate(&e->ev[i]);
@@ -1627,6 +1659,71 @@ void sjcs_index_destroy(sjcs_index* ix) {
   ix->aos.release();
   ix->ib.release();
   delete ix;
+}
+
+int sjcs_index_count_noalloc(sjcs_index* ix, const float* d_centrals, uint64_t n,
+                             const float* radiuses_sq, int nrads, uint64_t* d_counts) {
+  if (!ix) return SJCS_ERR_INVALID;
+  sjcs_engine* e = ix->e;
+  if (!ix->built) return fail(e, SJCS_ERR_STATE, "index not built (sjcs_index_build)");
+  if ((!d_centrals && n) || (!d_counts && n) || !radiuses_sq || nrads <= 0)
+    return fail(e, SJCS_ERR_INVALID, "null pointer or nrads < 1");
+  if (nrads > kMaxLdsShells)
+    return fail(e, SJCS_ERR_INVALID, "noalloc count: at most 1024 shells");
+  // scratch layout: shell table, thresholds, then per halo its Morton key,
+  // x, y, z and row (u32 each) and its u32 counts, each part 256-aligned
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t off_r2 = up(sizeof(ShellEntry) * kLutSize);
+  const size_t off_h = off_r2 + up(4 * (size_t)kMaxLdsShells);
+  const size_t hb = n <= kNoAllocScratchBytes ? up(4 * (size_t)n) : kNoAllocScratchBytes;
+  const size_t off_c = off_h + 5 * hb;
+  const double need = (double)off_c + 4.0 * (double)n * (double)nrads;
+  if (need > (double)kNoAllocScratchBytes)
+    return fail(e, SJCS_ERR_INVALID,
+                "noalloc count: " + std::to_string((uint64_t)need) +
+                    " bytes of device scratch needed, at most " +
+                    std::to_string(kNoAllocScratchBytes) +
+                    " (20 + 4*nrads bytes per halo plus 12 KiB)");
+  int rc = use_device(e);
+  if (rc != SJCS_OK) return rc;
+  if (!e->scratch) return fail(e, SJCS_ERR_STATE, "noalloc count: no device scratch");
+  if (n == 0) return SJCS_OK;
+  Shells v{};
+  ShellEntry lut[kLutSize];
+  int nlut = 1;
+  shell_table_into(radiuses_sq, nrads, &v, lut, &nlut);
+  if (!(v.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
+    return fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
+  char* base = static_cast<char*>(e->scratch);
+  uint32_t* hw = reinterpret_cast<uint32_t*>(base + off_h);
+  const size_t hwords = hb / 4;
+  HIPCALL(e, hipMemcpyAsync(base, lut, sizeof(ShellEntry) * (size_t)nlut, hipMemcpyHostToDevice,
+                            e->stream),
+          "hipMemcpy(shell table)");
+  HIPCALL(e, hipMemcpyAsync(base + off_r2, radiuses_sq, 4 * (size_t)nrads, hipMemcpyHostToDevice,
+                            e->stream),
+          "hipMemcpy(shells)");
+  // centres -> SoA + rows, counted in input order (no halo sort: no sort
+  // workspace), so the u32 counts add straight into the caller's u64 rows
+  HIPCALL(e, halo_keys(d_centrals, 12, n, hw, reinterpret_cast<float*>(hw + hwords),
+                       reinterpret_cast<float*>(hw + 2 * hwords),
+                       reinterpret_cast<float*>(hw + 3 * hwords), hw + 4 * hwords, e->stream),
+          "halo keys");
+  v.r2 = reinterpret_cast<const float*>(base + off_r2);
+  v.lut = reinterpret_cast<const ShellEntry*>(base);
+  HaloView hv;
+  hv.x = reinterpret_cast<const float*>(hw + hwords);
+  hv.y = reinterpret_cast<const float*>(hw + 2 * hwords);
+  hv.z = reinterpret_cast<const float*>(hw + 3 * hwords);
+  hv.orig = hw + 4 * hwords;
+  hv.n = n;
+  uint32_t* c32 = reinterpret_cast<uint32_t*>(base + off_c);
+  HIPCALL(e, count_halos(ix->ib.view(), hv, v, c32, false, nullptr, lanes(e, ix->n, v.rbox),
+                         seg_len(e), e->stream),
+          "noalloc count");
+  HIPCALL(e, widen_counts(c32, d_counts, n * (uint64_t)nrads, false, e->stream), "add counts");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  return SJCS_OK;
 }
 
 }  // extern "C"
