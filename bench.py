@@ -453,7 +453,8 @@ def main():
         best = eng.autotune(d_parts[0], n)
         tuned = {"wall_s": round(time.perf_counter() - t_at, 3), "best_ms": round(best, 3),
                  "chosen": {k: eng.get_option(k) for k in
-                            ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles")}}
+                            ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles",
+                             "tile_level", "tile_halos")}}
         log(f"[bench] autotune: {tuned}")
 
     # exact W of one step: an untimed pair-test pass per chunk
@@ -495,8 +496,10 @@ def main():
         # the plain one-halo-per-wavefront kernel on the same inputs, same
         # run: default cells, no tiles
         tuned_opts = {k_: eng.get_option(k_) for k_ in
-                      ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles")}
-        for k_ in ("cell_size_milli", "x_split", "lanes_per_halo", "segment"):
+                      ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles",
+                       "tile_level", "tile_halos")}
+        for k_ in ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tile_level",
+                   "tile_halos"):
             eng.set_option(k_, 0)
         eng.set_option("tiles", 0)
         eng.set_option("kernel", 1)

@@ -210,6 +210,19 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *   "piece_max"         most particles of a chunk indexed and counted as
  *                       one piece (0 = the most that 32-bit offsets and
  *                       counts allow; small values exercise the split)
+ *   "tile_level"        0 (default) = a tile is a run of at least 4 halos
+ *                       in one cell of the halos' 1024^3 Morton grid (about
+ *                       one unit); L = 1..6: a run of at least 2 halos in
+ *                       one cell of 2^L grid cells per side, so neighbouring
+ *                       halos share their candidates too.  Counts already
+ *                       accumulated are kept.
+ *   "tile_halos"        most halos per tile, 2 to 64 (0 = as many as the
+ *                       tile kernel's LDS histograms hold: 2048 / nrads);
+ *                       smaller tiles need less LDS per wavefront
+ *   "autotune_cache"    1 (default) = sjcs_autotune reuses the setting it
+ *                       found for the same workload in this process
+ * sjcs_get_option also reads "autotune_cache_hit": 1 when the last
+ * sjcs_autotune took its setting from that cache.
  * Returns SJCS_ERR_INVALID for an unknown key or value. */
 int sjcs_set_option(sjcs_engine *e, const char *key, int64_t value);
 
@@ -219,10 +232,15 @@ int sjcs_get_option(sjcs_engine *e, const char *key, int64_t *value);
 /* Autotuner (no reference counterpart): times the cell sort + count of the
  * chunk at d_xyz (device, n packed points) against the current halos and
  * shells under candidate settings -- lanes per halo, cell edge along y/z as
- * a fraction of r_box, x split, segment, tiles, by coordinate descent -- and
- * keeps the fastest as the engine's options (sjcs_get_option reads them).
- * The counts are not touched; the last chunk's index is discarded.
- * *best_ms (may be NULL) receives the winner's sort + count time. */
+ * a fraction of r_box, x split, segment, then halo tiles on or off, tile
+ * level and halos per tile, by coordinate descent -- and keeps the fastest
+ * as the engine's options (sjcs_get_option reads them).  The setting is
+ * remembered for the workload -- particle and halo counts, the thresholds,
+ * and histograms of how particles and halos fill the box -- and reused
+ * without timing when the same workload is tuned again in this process
+ * ("autotune_cache").  The counts are not touched; the last chunk's index
+ * is discarded.  *best_ms (may be NULL) receives the winner's sort + count
+ * time. */
 int sjcs_autotune(sjcs_engine *e, const float *d_xyz, uint64_t n, float *best_ms);
 
 /* Milliseconds the device spent on the last chunk, by phase (HIP events):

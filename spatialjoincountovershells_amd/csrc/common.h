@@ -102,12 +102,13 @@ struct IndexView {
   uint64_t n;
 };
 
-// Halos, sorted by cell key for locality; orig[i] is the count row of i.
+// Halos in a spatially coherent order; halo i's counts are row slot[i] of
+// the counts (row i when slot is null).
 struct HaloView {
   const float* x;
   const float* y;
   const float* z;
-  const uint32_t* orig;
+  const uint32_t* slot;
   uint64_t n;
 };
 

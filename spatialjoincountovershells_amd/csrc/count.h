@@ -37,9 +37,13 @@ hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh
 hipError_t widen_counts(const uint32_t* c32, uint64_t* c64, uint64_t total, bool store,
                         hipStream_t s);
 
-// dst[orig[i]*K + k] = c64[i*K + k] + c32[i*K + k] (a null source adds 0):
-// counts from sorted halo order back to the halos' input order.
-hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* orig,
+// inv[orig[i]] = i for i < n (orig a permutation of [0, n)).
+hipError_t invert_rows(const uint32_t* orig, uint32_t* inv, uint64_t n, hipStream_t s);
+
+// dst[r*K + k] = c64[inv[r]*K + k] + c32[inv[r]*K + k] (a null source adds
+// 0): counts from sorted halo order back to the halos' input order, inv[r]
+// being input row r's sorted position (invert_rows).  Requires K*64 < 2^32.
+hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* inv,
                           uint64_t* dst, uint64_t n, int K, hipStream_t s);
 
 }  // namespace sjcs

@@ -161,11 +161,12 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     // counts are kept in the sorted halo order (rows of neighbouring halos
     // are neighbours in memory); they are put back in input order once, when
     // they are read out
-    const uint64_t row = hs;
+    uint64_t row = hs;
     if (active) {
       cx = hv.x[hs];
       cy = hv.y[hs];
       cz = hv.z[hs];
+      if (hv.slot) row = hv.slot[hs];
     }
     if (kLds && !kInbox) {
       for (int k = gl; k < K; k += GROUP) bins[k] = 0u;
@@ -421,21 +422,45 @@ __global__ void widen_counts_kernel(const uint32_t* __restrict__ c32, uint64_t* 
     c64[i] = (store ? 0ull : c64[i]) + c32[i];
 }
 
-// dst[orig[i] * K + k] = c64[i * K + k] + c32[i * K + k] (either may be
-// absent): rows back in input order, one row per wavefront at a time.
-__global__ void readout_counts_kernel(const uint32_t* __restrict__ c32,
-                                      const uint64_t* __restrict__ c64,
-                                      const uint32_t* __restrict__ orig, uint64_t* __restrict__ dst,
-                                      uint64_t n, uint32_t K) {
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-  const int lane = threadIdx.x & 63;
-  for (uint64_t i = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n;
-       i += waves) {
-    uint64_t* d = dst + (uint64_t)orig[i] * K;
-    for (uint32_t k = lane; k < K; k += 64) {
-      const uint64_t a = c64 ? c64[i * K + k] : 0ull;
-      const uint64_t b = c32 ? c32[i * K + k] : 0u;
-      d[k] = a + b;
+// inv[orig[i]] = i: for each input row, its position in sorted order.
+__global__ void invert_rows_kernel(const uint32_t* __restrict__ orig, uint32_t* __restrict__ inv,
+                                   uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    inv[orig[i]] = (uint32_t)i;
+}
+
+// dst[r * K + k] = c64[inv[r] * K + k] + c32[inv[r] * K + k] (either source
+// may be absent): the rows back in input order.  A block writes
+// kReadoutRows consecutive output rows, its lanes walking their
+// kReadoutRows * K words in order, so the u64 stores are contiguous and each
+// source row is one contiguous read of K words.
+constexpr uint32_t kReadoutRows = 64;
+constexpr int kReadoutUnroll = 4;
+
+__global__ __launch_bounds__(256) void readout_counts_kernel(
+    const uint32_t* __restrict__ c32, const uint64_t* __restrict__ c64,
+    const uint32_t* __restrict__ inv, uint64_t* __restrict__ dst, uint64_t n, uint32_t K) {
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kReadoutRows; r0 < n;
+       r0 += (uint64_t)gridDim.x * kReadoutRows) {
+    const uint32_t rows = (uint32_t)min((uint64_t)kReadoutRows, n - r0);
+    const uint32_t m = rows * K;
+    uint64_t* d = dst + r0 * K;
+    for (uint32_t o0 = threadIdx.x; o0 < m; o0 += 256 * kReadoutUnroll) {
+      uint64_t v[kReadoutUnroll];
+#pragma unroll
+      for (int u = 0; u < kReadoutUnroll; ++u) {
+        const uint32_t o = o0 + 256u * u;
+        v[u] = 0;
+        if (o < m) {
+          const uint32_t r = o / K, k = o - r * K;
+          const uint64_t src = (uint64_t)inv[r0 + r] * K + k;
+          v[u] = (c64 ? c64[src] : 0ull) + (c32 ? c32[src] : 0u);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kReadoutUnroll; ++u)
+        if (o0 + 256u * u < m) d[o0 + 256u * u] = v[u];
     }
   }
 }
@@ -451,12 +476,20 @@ hipError_t widen_counts(const uint32_t* c32, uint64_t* c64, uint64_t total, bool
   return hipGetLastError();
 }
 
-hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* orig,
+hipError_t invert_rows(const uint32_t* orig, uint32_t* inv, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = ceil_div(n, 256);
+  if (blocks > 65536) blocks = 65536;
+  invert_rows_kernel<<<(uint32_t)blocks, 256, 0, s>>>(orig, inv, n);
+  return hipGetLastError();
+}
+
+hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* inv,
                           uint64_t* dst, uint64_t n, int K, hipStream_t s) {
   if (n == 0 || K <= 0) return hipSuccess;
-  uint64_t blocks = ceil_div(n, 4);
-  if (blocks > 65536) blocks = 65536;
-  readout_counts_kernel<<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, orig, dst, n, (uint32_t)K);
+  uint64_t blocks = ceil_div(n, kReadoutRows);
+  if (blocks > 16384) blocks = 16384;
+  readout_counts_kernel<<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, inv, dst, n, (uint32_t)K);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -78,33 +79,49 @@ struct ShellState {
   Shells view{};
 };
 
-// Halos in Morton order, with runs of at least kTileMin halos sharing a
-// Morton cell moved to the front ([0, n_tiled)); `runs` lists them.  Tiles
-// for a shell count are cut from the runs (tile_cap halos at most).
+// Halos in Morton order (30-bit keys of a 1024^3 grid): a halo's count row
+// is its position in that order, fixed for the halo set.  Tiles are runs of
+// halos whose keys agree above their low 3*level bits -- level 0: one Morton
+// cell (halos within about a unit, runs of at least kTileMin); level L: a
+// cell of 2^L grid cells per side (runs of at least 2).  For the count the
+// tiled runs go first, in order, then every other halo ("working order",
+// slot[i] = count row of working halo i); runs longer than tile_cap are cut
+// into tiles.
 constexpr uint32_t kTileMin = 4;
+constexpr int kTileLevelMax = 6;
 
 struct HaloState {
   uint64_t n = 0;
-  DevBuf x, y, z, row;
+  DevBuf bx, by, bz, brow, bkey;  // Morton order: centres, input rows, keys
+  DevBuf inv;                      // input row -> count row
+  bool runs0 = false;              // some kTileMin halos share a Morton key
+  std::vector<uint32_t> hkeys;     // host copy of bkey, fetched for tiling
+  int level = -1;                  // tile level of the working order
+  bool permuted = false;           // working order differs from Morton order
+  DevBuf x, y, z, slot;            // working order (when permuted)
   uint64_t n_tiled = 0;
-  std::vector<uint32_t> run_start, run_len;
+  std::vector<uint32_t> run_start, run_len;  // runs, in working positions
   DevBuf tstart, tlen;  // tiles for tile_cap == tcap
   DevBuf twork;         // count_tiles workspace
   uint32_t ntiles = 0;
+  uint32_t tmax = 0;  // longest tile
   int tcap = 0;
-  void release() {
-    x.release();
-    y.release();
-    z.release();
-    row.release();
-    tstart.release();
-    tlen.release();
-    twork.release();
+  void clear_tiles() {
+    permuted = false;
     n_tiled = 0;
     run_start.clear();
     run_len.clear();
     ntiles = 0;
+    tmax = 0;
     tcap = 0;
+  }
+  void release() {
+    for (DevBuf* d : {&bx, &by, &bz, &brow, &bkey, &inv, &x, &y, &z, &slot, &tstart, &tlen, &twork})
+      d->release();
+    hkeys.clear();
+    runs0 = false;
+    level = -1;
+    clear_tiles();
   }
 };
 
@@ -284,6 +301,10 @@ struct sjcs_engine {
   int64_t piece_max = 0;   // 0: pieces as large as 32-bit offsets and counts allow
   int64_t graph = 0;       // 1: each piece's sort + count replayed from a graph
   int64_t tiles = 1;       // 1: halo tiles share their candidates (ascending shells)
+  int64_t tile_level = 0;  // halos tiled together: one Morton cell of 2^level grid cells
+  int64_t tile_halos = 0;  // most halos per tile (0: as many as the LDS histograms hold)
+  int64_t autotune_cache = 1;  // 1: sjcs_autotune reuses the setting found for a workload
+  bool autotune_hit = false;   // the last sjcs_autotune took its setting from the cache
 
   // Captured sort + count of a piece, per slot and store/accumulate; valid
   // while input, size and `epoch` match.  `epoch` changes whenever a pointer
@@ -454,12 +475,77 @@ int make_shells(sjcs_engine* e, ShellState& st, const float* r2, int nrads) {
   return SJCS_OK;
 }
 
-// Halo centres from a device AoS (packed xyz) -> Morton-sorted SoA + rows.
+// Working order of the halos for tile level `level` (see HaloState):
+// runs found on the host from the Morton keys, the tiled runs moved to the
+// front on the device.  Leaves the Morton order as working order when no
+// run qualifies.  Callers make sure no count in flight reads the halos.
+int apply_tiles(sjcs_engine* e, HaloState& hs, int level) {
+  hs.clear_tiles();
+  hs.level = level;
+  const uint64_t n = hs.n;
+  const uint32_t minrun = level == 0 ? kTileMin : 2;
+  if (n < minrun || (level == 0 && !hs.runs0)) return SJCS_OK;
+  if (hs.hkeys.size() != n) {
+    hs.hkeys.resize(n);
+    HIPCALL(e, hipMemcpy(hs.hkeys.data(), hs.bkey.p, 4 * n, hipMemcpyDeviceToHost), "halo keys");
+  }
+  const int shift = 3 * level;
+  const std::vector<uint32_t>& keys = hs.hkeys;
+  std::vector<uint32_t> rs, rl;
+  uint64_t ntiled = 0;
+  for (uint64_t i = 0; i < n;) {
+    uint64_t j = i + 1;
+    while (j < n && (keys[j] >> shift) == (keys[i] >> shift)) ++j;
+    if (j - i >= minrun) {
+      rs.push_back((uint32_t)i);
+      rl.push_back((uint32_t)(j - i));
+      ntiled += j - i;
+    }
+    i = j;
+  }
+  if (!ntiled) return SJCS_OK;
+  std::vector<uint32_t> perm;
+  perm.reserve(n);
+  std::vector<uint32_t> nstart(rs.size());
+  for (size_t r = 0; r < rs.size(); ++r) {
+    nstart[r] = (uint32_t)perm.size();
+    for (uint32_t k = 0; k < rl[r]; ++k) perm.push_back(rs[r] + k);
+  }
+  for (uint64_t i = 0, r = 0; i < n; ++i) {
+    if (r < rs.size() && i == rs[r]) {
+      i += rl[r] - 1;
+      ++r;
+      continue;
+    }
+    perm.push_back((uint32_t)i);
+  }
+  const size_t b = 4 * n;
+  HIPCALL(e, ensure(hs.x, b), "hipMalloc(halos)");
+  HIPCALL(e, ensure(hs.y, b), "hipMalloc(halos)");
+  HIPCALL(e, ensure(hs.z, b), "hipMalloc(halos)");
+  HIPCALL(e, ensure(hs.slot, b), "hipMalloc(halos)");
+  // slot[i] = perm[i]: a halo's count row is its Morton position
+  HIPCALL(e, hipMemcpy(hs.slot.p, perm.data(), b, hipMemcpyHostToDevice), "hipMemcpy(halos)");
+  HIPCALL(e, permute_halos(hs.bx.as<float>(), hs.by.as<float>(), hs.bz.as<float>(), nullptr,
+                           hs.x.as<float>(), hs.y.as<float>(), hs.z.as<float>(), nullptr,
+                           hs.slot.as<uint32_t>(), n, e->stream),
+          "halo tiling");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  hs.permuted = true;
+  hs.n_tiled = ntiled;
+  hs.run_start.swap(nstart);
+  hs.run_len.swap(rl);
+  return SJCS_OK;
+}
+
+// Halo centres from a device AoS (3 FP32 every `stride` bytes) -> Morton
+// order (count rows), input rows, the read-out's inverse; with `tiles`, the
+// working order for the engine's tile level.
 int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride, uint64_t n,
                bool tiles, bool timed) {
   if (n > 0xffffffffull) return fail(e, SJCS_ERR_INVALID, "at most 2^32-1 halos");
   const size_t b = sizeof(uint32_t) * (n ? n : 1);
-  DevBuf key[2], x[2], y[2], z[2], row[2];
+  DevBuf key[2], x[2], y[2], z[2], row[2], inv;
   int rc = SJCS_OK;
   hipError_t h = hipSuccess;
   for (int i = 0; i < 2 && h == hipSuccess; ++i) {
@@ -469,6 +555,7 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
     if (h == hipSuccess) h = ensure(z[i], b);
     if (h == hipSuccess) h = ensure(row[i], b);
   }
+  if (h == hipSuccess) h = ensure(inv, b);
   if (h == hipSuccess) rc = ensure_ws(e, radix_workspace_words(n));
   if (h == hipSuccess && rc == SJCS_OK && timed) h = hipEventRecord(e->hev[0], e->stream);
   if (h == hipSuccess && rc == SJCS_OK)
@@ -482,7 +569,10 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
         {x[1].as<uint32_t>(), y[1].as<uint32_t>(), z[1].as<uint32_t>(), row[1].as<uint32_t>()}};
     h = radix_sort_pairs(keys, pay, 4, n, 30, e->ws.as<uint32_t>(), e->stream, &sel);
   }
-  // the host looks for runs only when the device saw one
+  // the read-out gathers each input row from its count row
+  if (h == hipSuccess && rc == SJCS_OK)
+    h = invert_rows(row[sel].as<uint32_t>(), inv.as<uint32_t>(), n, e->stream);
+  // the host looks for runs of one Morton key only when the device saw one
   uint32_t runs = 0;
   if (h == hipSuccess && rc == SJCS_OK && tiles && n >= kTileMin) {
     h = ensure(e->inbox, 16);
@@ -493,76 +583,26 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
   }
   if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
   if (h != hipSuccess && rc == SJCS_OK) rc = hip_fail(e, h, "halo sort");
-  // runs of >= kTileMin halos with one Morton key go first, in order
-  std::vector<uint32_t> rs, rl;
-  uint64_t ntiled = 0;
-  if (rc == SJCS_OK && tiles && n >= kTileMin && runs) {
-    std::vector<uint32_t> keys(n);
-    h = hipMemcpy(keys.data(), key[sel].p, 4 * n, hipMemcpyDeviceToHost);
-    if (h != hipSuccess) rc = hip_fail(e, h, "halo keys");
-    for (uint64_t i = 0; rc == SJCS_OK && i < n;) {
-      uint64_t j = i + 1;
-      while (j < n && keys[j] == keys[i]) ++j;
-      if (j - i >= kTileMin) {
-        rs.push_back((uint32_t)i);
-        rl.push_back((uint32_t)(j - i));
-        ntiled += j - i;
-      }
-      i = j;
-    }
-    if (rc == SJCS_OK && ntiled) {
-      std::vector<uint32_t> perm;
-      perm.reserve(n);
-      std::vector<uint32_t> nstart(rs.size());
-      for (size_t r = 0; r < rs.size(); ++r) {
-        nstart[r] = (uint32_t)perm.size();
-        for (uint32_t k = 0; k < rl[r]; ++k) perm.push_back(rs[r] + k);
-      }
-      size_t r = 0;
-      for (uint64_t i = 0; i < n; ++i) {
-        if (r < rs.size() && i == rs[r]) {
-          i += rl[r] - 1;
-          ++r;
-          continue;
-        }
-        perm.push_back((uint32_t)i);
-      }
-      rs.swap(nstart);
-      DevBuf dperm;
-      h = ensure(dperm, 4 * n);
-      if (h == hipSuccess) h = hipMemcpy(dperm.p, perm.data(), 4 * n, hipMemcpyHostToDevice);
-      const int o = sel ^ 1;
-      if (h == hipSuccess)
-        h = permute_halos(x[sel].as<float>(), y[sel].as<float>(), z[sel].as<float>(),
-                          row[sel].as<uint32_t>(), x[o].as<float>(), y[o].as<float>(),
-                          z[o].as<float>(), row[o].as<uint32_t>(), dperm.as<uint32_t>(), n,
-                          e->stream);
-      if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
-      dperm.release();
-      if (h != hipSuccess) rc = hip_fail(e, h, "halo tiling");
-      sel = o;
-    }
+  if (rc == SJCS_OK) {
+    hs.release();
+    hs.n = n;
+    hs.runs0 = runs != 0;
+    hs.bx = x[sel];
+    hs.by = y[sel];
+    hs.bz = z[sel];
+    hs.brow = row[sel];
+    hs.bkey = key[sel];
+    hs.inv = inv;
+    x[sel] = y[sel] = z[sel] = row[sel] = key[sel] = inv = DevBuf();
   }
+  if (rc == SJCS_OK && tiles) rc = apply_tiles(e, hs, (int)e->tile_level);
   if (rc == SJCS_OK && timed) {
     h = hipEventRecord(e->hev[1], e->stream);
     if (h != hipSuccess) rc = hip_fail(e, h, "hipEventRecord");
     e->halo_timed = rc == SJCS_OK;
   }
-  if (rc == SJCS_OK) {
-    hs.release();
-    hs.n = n;
-    hs.n_tiled = ntiled;
-    hs.run_start.swap(rs);
-    hs.run_len.swap(rl);
-    hs.x = x[sel];
-    hs.y = y[sel];
-    hs.z = z[sel];
-    hs.row = row[sel];
-    x[sel] = DevBuf();
-    y[sel] = DevBuf();
-    z[sel] = DevBuf();
-    row[sel] = DevBuf();
-  }
+  // the sort's other buffers go after the timed part (a free waits for the
+  // device)
   for (int i = 0; i < 2; ++i) {
     key[i].release();
     x[i].release();
@@ -570,6 +610,7 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
     z[i].release();
     row[i].release();
   }
+  inv.release();
   return rc;
 }
 
@@ -679,12 +720,13 @@ int build_index(sjcs_engine* e, IndexBuffers& ib, const ParticleSource& src, uin
   return enqueue_index(e, ib, src, n, g, st);
 }
 
+// The halos in working order (see HaloState).
 HaloView halo_view(const HaloState& h) {
   HaloView v;
-  v.x = h.x.as<float>();
-  v.y = h.y.as<float>();
-  v.z = h.z.as<float>();
-  v.orig = h.row.as<uint32_t>();
+  v.x = (h.permuted ? h.x : h.bx).as<float>();
+  v.y = (h.permuted ? h.y : h.by).as<float>();
+  v.z = (h.permuted ? h.z : h.bz).as<float>();
+  v.slot = h.permuted ? h.slot.as<uint32_t>() : nullptr;
   v.n = h.n;
   return v;
 }
@@ -749,7 +791,8 @@ int fold_counts(sjcs_engine* e) {
 int ensure_tiles(sjcs_engine* e, bool* use) {
   *use = false;
   HaloState& hs = e->halos;
-  const int cap = tile_cap(e->shells.nrads);
+  int cap = tile_cap(e->shells.nrads);
+  if (e->tile_halos > 0 && e->tile_halos < cap) cap = (int)e->tile_halos;
   if (!e->tiles || !e->shells.view.sorted || hs.n_tiled == 0 || cap < 2) return SJCS_OK;
   *use = true;
   if (hs.tcap == cap) return SJCS_OK;
@@ -773,6 +816,7 @@ int ensure_tiles(sjcs_engine* e, bool* use) {
   HIPCALL(e, hipMemcpy(hs.tstart.p, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice), "hipMemcpy");
   HIPCALL(e, hipMemcpy(hs.tlen.p, tl.data(), 4 * tl.size(), hipMemcpyHostToDevice), "hipMemcpy");
   hs.ntiles = (uint32_t)ts.size();
+  hs.tmax = tl.empty() ? 0u : *std::max_element(tl.begin(), tl.end());
   hs.tcap = cap;
   return SJCS_OK;
 }
@@ -789,17 +833,18 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint
   tv.start = e->halos.tstart.as<uint32_t>();
   tv.len = e->halos.tlen.as<uint32_t>();
   tv.n = e->halos.ntiles;
+  tv.max_len = e->halos.tmax;
   hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox,
                              e->halos.twork.as<uint32_t>(), st);
   if (h != hipSuccess) return h;
+  // the other halos, after the tiled runs; slot[] names their count rows
   const uint64_t nt = e->halos.n_tiled;
   hv.x += nt;
   hv.y += nt;
   hv.z += nt;
-  hv.orig += nt;
+  hv.slot += nt;
   hv.n -= nt;
-  return count_halos(ix, hv, sh, counts ? counts + nt * (uint64_t)sh.nrads : nullptr, accumulate,
-                     inbox, lanes(e, ix.n), seg_len(e), st);
+  return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e, ix.n), seg_len(e), st);
 }
 
 int piece_done(sjcs_engine* e, uint64_t n);
@@ -1262,7 +1307,7 @@ int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint6
     HIPCALL(e, hipEventRecord(e->rev[0], e->stream), "hipEventRecord");
     HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
                               e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
-                              e->halos.row.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
+                              e->halos.inv.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
                               e->shells.nrads, e->stream),
             "read out counts");
     HIPCALL(e, hipEventRecord(e->rev[1], e->stream), "hipEventRecord");
@@ -1305,7 +1350,9 @@ int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
               {"segment", e->segment},                 {"tiles", e->tiles},
               {"graph", e->graph},                     {"fold_limit", e->fold_limit},
               {"overlap", e->overlap},                 {"index_sort", e->index_sort},
-              {"piece_max", e->piece_max}};
+              {"piece_max", e->piece_max},             {"tile_level", e->tile_level},
+              {"tile_halos", e->tile_halos},           {"autotune_cache", e->autotune_cache},
+              {"autotune_cache_hit", e->autotune_hit ? 1 : 0}};
   for (const auto& o : opts)
     if (strcmp(key, o.k) == 0) {
       *value = o.v;
@@ -1313,6 +1360,68 @@ int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
     }
   return fail(e, SJCS_ERR_INVALID, std::string("unknown option ") + key);
 }
+
+}  // extern "C"
+
+namespace {
+
+struct TuneSetting {
+  int64_t cell, xs, lanes, seg, tiles, level, thalos;
+};
+
+// Autotuned settings of the workloads seen by this process (all engines).
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<TuneSetting, float>> g_tune_cache;
+
+// The workload a tuned setting is kept for: particle and halo counts, the
+// thresholds, and how particles and halos fill the box (kOccCells^3 cell
+// occupancy histograms, each bin's share of the cells quantized to 1/255).
+// ws: occupancy_workspace_words() + 2 * kOccBins words of device memory.
+int workload_key(sjcs_engine* e, const float* d_xyz, uint64_t n, uint32_t* ws, std::string* key) {
+  uint32_t* hist = ws + occupancy_workspace_words();
+  const HaloState& hs = e->halos;
+  HIPCALL(e, occupancy_histogram(d_xyz, d_xyz + 1, d_xyz + 2, 3, n, ws, hist, e->stream),
+          "occupancy");
+  HIPCALL(e, occupancy_histogram(hs.bx.as<float>(), hs.by.as<float>(), hs.bz.as<float>(), 1, hs.n,
+                                 ws, hist + kOccBins, e->stream),
+          "occupancy");
+  uint32_t h[2 * kOccBins];
+  HIPCALL(e, hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, e->stream), "hipMemcpy");
+  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+  char buf[96];
+  snprintf(buf, sizeof(buf), "np=%llu nh=%llu K=%d runs=%d r2=", (unsigned long long)n,
+           (unsigned long long)hs.n, e->shells.nrads, hs.runs0 ? 1 : 0);
+  std::string k(buf);
+  for (float v : e->shells.r2) {
+    snprintf(buf, sizeof(buf), "%08x", float_bits(v));
+    k += buf;
+  }
+  k += " occ=";
+  const double cells = (double)occupancy_workspace_words();
+  for (int i = 0; i < 2 * kOccBins; ++i) {
+    snprintf(buf, sizeof(buf), "%02x", (unsigned)lround(255.0 * (double)h[i] / cells));
+    k += buf;
+  }
+  *key = k;
+  return SJCS_OK;
+}
+
+void apply_setting(sjcs_engine* e, const TuneSetting& st) {
+  e->cell_size_milli = st.cell;
+  e->x_split = st.xs;
+  e->lanes_per_halo = st.lanes;
+  e->segment = st.seg;
+  e->tiles = st.tiles;
+  e->tile_level = st.level;
+  e->tile_halos = st.thalos;
+  e->kernel = 0;
+  e->halos.tcap = 0;  // tiles cut again for this setting
+  ++e->epoch;
+}
+
+}  // namespace
+
+extern "C" {
 
 int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms) {
   int rc = check_ready(e);
@@ -1322,23 +1431,37 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
   rc = quiesce(e);
   if (rc != SJCS_OK) return rc;
   const size_t nc = (size_t)e->halos.n * (size_t)e->shells.nrads;
+  const size_t trial_words = std::max<size_t>(nc ? nc : 1, occupancy_workspace_words() + 2 * kOccBins);
   DevBuf trial_counts;
-  HIPCALL(e, ensure(trial_counts, 4 * (nc ? nc : 1)), "hipMalloc(autotune)");
-  struct Setting {
-    int64_t cell, xs, lanes, seg, tiles;
-  };
+  HIPCALL(e, ensure(trial_counts, 4 * trial_words), "hipMalloc(autotune)");
   const float rbox = e->shells.view.rbox;
-  Setting cur{e->cell_size_milli, e->x_split, e->lanes_per_halo, e->segment, e->tiles};
+  TuneSetting cur{e->cell_size_milli, e->x_split, e->lanes_per_halo, e->segment, e->tiles,
+                  e->tile_level, e->tile_halos};
   if (e->kernel == 1) cur.lanes = 64;
-  // one timed sort + count (second of two runs) of this chunk with setting st
-  auto run = [&](const Setting& st, float* ms) -> int {
-    e->cell_size_milli = st.cell;
-    e->x_split = st.xs;
-    e->lanes_per_halo = st.lanes;
-    e->segment = st.seg;
-    e->tiles = st.tiles;
-    e->kernel = 0;
-    ++e->epoch;
+  std::string key;
+  if (e->autotune_cache) {
+    rc = workload_key(e, d_xyz, n, trial_counts.as<uint32_t>(), &key);
+    if (rc != SJCS_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      apply_setting(e, it->second.first);
+      if (e->halos.level != (int)e->tile_level) rc = apply_tiles(e, e->halos, (int)e->tile_level);
+      if (rc == SJCS_OK && best_ms) *best_ms = it->second.second;
+      e->autotune_hit = true;
+      return rc;
+    }
+  }
+  e->autotune_hit = false;
+  // one timed sort + count (second of two runs) of this chunk with setting
+  // st; the counts go to a scratch buffer (count rows do not depend on the
+  // tiling, so the totals are left as they are)
+  auto run = [&](const TuneSetting& st, float* ms) -> int {
+    apply_setting(e, st);
+    if (e->halos.level != (int)st.level) {
+      int r = apply_tiles(e, e->halos, (int)st.level);
+      if (r != SJCS_OK) return r;
+    }
     const Grid g = choose_grid(n, rbox, e->cell_size_milli, e->x_split);
     bool use_tiles = false;
     int r = ensure_tiles(e, &use_tiles);
@@ -1347,8 +1470,8 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
       HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
       r = build_index(e, e->ibs[0], packed(d_xyz), n, g, e->stream);
       if (r != SJCS_OK) return r;
-      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, trial_counts.as<uint32_t>(), false, nullptr,
-                             e->stream),
+      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, trial_counts.as<uint32_t>(), false,
+                             nullptr, e->stream),
               "autotune count");
       HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
       HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
@@ -1358,9 +1481,7 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
   };
   float best = 0.0f;
   rc = run(cur, &best);
-  // coordinate descent: lanes, y/z cell edge (as a fraction of r_box), x
-  // split, segment, tiles
-  auto try_set = [&](Setting st) -> int {
+  auto try_set = [&](TuneSetting st) -> int {
     float ms = 0.0f;
     int r = run(st, &ms);
     if (r == SJCS_OK && ms < best) {
@@ -1369,31 +1490,69 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
     }
     return r;
   };
+  // coordinate descent: lanes, y/z cell edge (as a fraction of r_box), x
+  // split, segment, then the halo tiles -- whether, how far apart their
+  // halos may lie (tile level) and how many a tile holds
   for (int64_t l : {16, 64})
-    if (rc == SJCS_OK && l != cur.lanes) rc = try_set({cur.cell, cur.xs, l, cur.seg, cur.tiles});
+    if (rc == SJCS_OK && l != cur.lanes) {
+      TuneSetting st = cur;
+      st.lanes = l;
+      rc = try_set(st);
+    }
   if (rc == SJCS_OK && rbox > 0.0f && isfinite(rbox)) {
     for (double f : {0.3, 0.45, 0.6, 0.75, 0.9, 1.2}) {
       const int64_t milli = std::max<int64_t>(977, (int64_t)(f * rbox * 1000.0));
-      if (rc == SJCS_OK && milli != cur.cell) rc = try_set({milli, cur.xs, cur.lanes, cur.seg, cur.tiles});
+      if (rc == SJCS_OK && milli != cur.cell) {
+        TuneSetting st = cur;
+        st.cell = milli;
+        rc = try_set(st);
+      }
     }
   }
   for (int64_t x : {1, 2, 4, 8})
-    if (rc == SJCS_OK && x != cur.xs) rc = try_set({cur.cell, x, cur.lanes, cur.seg, cur.tiles});
+    if (rc == SJCS_OK && x != cur.xs) {
+      TuneSetting st = cur;
+      st.xs = x;
+      rc = try_set(st);
+    }
   for (int64_t sg : {4, 8})
-    if (rc == SJCS_OK && sg != cur.seg) rc = try_set({cur.cell, cur.xs, cur.lanes, sg, cur.tiles});
-  if (rc == SJCS_OK && e->halos.n_tiled)
-    rc = try_set({cur.cell, cur.xs, cur.lanes, cur.seg, cur.tiles ? 0 : 1});
+    if (rc == SJCS_OK && sg != cur.seg) {
+      TuneSetting st = cur;
+      st.seg = sg;
+      rc = try_set(st);
+    }
+  const int cap = tile_cap(e->shells.nrads);
+  if (rc == SJCS_OK && e->shells.view.sorted && cap >= 2) {
+    for (int64_t lv : {0, 1, 2, 3, 4})
+      if (rc == SJCS_OK && (lv != cur.level || !cur.tiles)) {
+        TuneSetting st = cur;
+        st.level = lv;
+        st.tiles = 1;
+        rc = try_set(st);
+      }
+    if (rc == SJCS_OK && cur.tiles) {
+      TuneSetting st = cur;
+      st.tiles = 0;
+      rc = try_set(st);
+    }
+    if (cur.tiles)
+      for (int64_t th : {8, 16, 32, 0})
+        if (rc == SJCS_OK && th < cap && th != cur.thalos) {
+          TuneSetting st = cur;
+          st.thalos = th;
+          rc = try_set(st);
+        }
+  }
   // keep the winner; the index buffers of slot 0 no longer hold the last chunk
-  e->cell_size_milli = cur.cell;
-  e->x_split = cur.xs;
-  e->lanes_per_halo = cur.lanes;
-  e->segment = cur.seg;
-  e->tiles = cur.tiles;
-  e->kernel = 0;
-  ++e->epoch;
+  apply_setting(e, cur);
+  if (rc == SJCS_OK && e->halos.level != (int)cur.level) rc = apply_tiles(e, e->halos, (int)cur.level);
   e->last = -1;
   e->timed = false;
   trial_counts.release();
+  if (rc == SJCS_OK && e->autotune_cache) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = std::make_pair(cur, best);
+  }
   if (rc == SJCS_OK && best_ms) *best_ms = best;
   return rc;
 }
@@ -1409,6 +1568,26 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
   if (strcmp(key, "tiles") == 0) {
     if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "tiles: 0 or 1");
     e->tiles = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "tile_level") == 0) {
+    if (value < 0 || value > kTileLevelMax)
+      return fail(e, SJCS_ERR_INVALID, "tile_level: 0 to " + std::to_string(kTileLevelMax));
+    e->tile_level = value;
+    // count rows stay the Morton positions: counts in progress are kept
+    if (e->have_halos && e->halos.level != (int)value) return apply_tiles(e, e->halos, (int)value);
+    return SJCS_OK;
+  }
+  if (strcmp(key, "autotune_cache") == 0) {
+    if (value < 0 || value > 1) return fail(e, SJCS_ERR_INVALID, "autotune_cache: 0 or 1");
+    e->autotune_cache = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "tile_halos") == 0) {
+    if (value < 0 || value == 1 || value > kTileMaxHalos)
+      return fail(e, SJCS_ERR_INVALID, "tile_halos: 0 or 2 to " + std::to_string(kTileMaxHalos));
+    e->tile_halos = value;
+    e->halos.tcap = 0;  // tiles are cut again at the next count
     return SJCS_OK;
   }
   if (strcmp(key, "graph") == 0) {
@@ -1631,7 +1810,7 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
       h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint32_t>(), false,
                       nullptr, lanes(e, ix->n, sh.view.rbox), seg_len(e), e->stream);
     if (h == hipSuccess)
-      h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.row.as<uint32_t>(),
+      h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.inv.as<uint32_t>(),
                          ucounts.as<uint64_t>(), n, nrads, e->stream);
     if (h == hipSuccess) {
       host.resize(nc);
@@ -1715,7 +1894,7 @@ int sjcs_index_count_noalloc(sjcs_index* ix, const float* d_centrals, uint64_t n
   hv.x = reinterpret_cast<const float*>(hw + hwords);
   hv.y = reinterpret_cast<const float*>(hw + 2 * hwords);
   hv.z = reinterpret_cast<const float*>(hw + 3 * hwords);
-  hv.orig = hw + 4 * hwords;
+  hv.slot = nullptr;  // input order: halo i's counts are row i
   hv.n = n;
   uint32_t* c32 = reinterpret_cast<uint32_t*>(base + off_c);
   HIPCALL(e, count_halos(ix->ib.view(), hv, v, c32, false, nullptr, lanes(e, ix->n, v.rbox),
@@ -1908,7 +2087,7 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   hv.x = sl.x[sel].as<float>();
   hv.y = sl.y[sel].as<float>();
   hv.z = sl.z[sel].as<float>();
-  hv.orig = sl.row[sel].as<uint32_t>();
+  hv.slot = nullptr;  // count rows in sorted order
   hv.n = n;
   r.shells.r2 = reinterpret_cast<float*>(sl.xyz.as<char>() + off_r2);
   r.shells.lut = reinterpret_cast<ShellEntry*>(sl.xyz.as<char>() + off_lut);
@@ -1917,7 +2096,10 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   const int lanes_auto = !((double)ix.n * side * side * side <= 4096.0) ? 64 : 16;
   SRVCALL(count_halos(ix, hv, r.shells, sl.c32.as<uint32_t>(), false, nullptr, lanes_auto, 4, sl.st),
           "count");
-  SRVCALL(readout_counts(sl.c32.as<uint32_t>(), nullptr, hv.orig, sl.c64.as<uint64_t>(), n, K, sl.st),
+  // the other key buffer is free after the sort: it takes the inverse rows
+  uint32_t* inv = sl.key[sel ^ 1].as<uint32_t>();
+  SRVCALL(invert_rows(sl.row[sel].as<uint32_t>(), inv, n, sl.st), "read out");
+  SRVCALL(readout_counts(sl.c32.as<uint32_t>(), nullptr, inv, sl.c64.as<uint64_t>(), n, K, sl.st),
           "read out");
   if (nc)
     SRVCALL(hipMemcpyAsync(sl.pin, sl.c64.p, 8 * nc, hipMemcpyDeviceToHost, sl.st), "hipMemcpy");

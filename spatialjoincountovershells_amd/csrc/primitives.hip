@@ -1105,4 +1105,49 @@ hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n, uint32_t* cel
   return inclusive_max_u32(cell_end, cell_end, ncells, ws, s);
 }
 
+
+namespace {
+
+__global__ void occupancy_count_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                       const float* __restrict__ pz, uint32_t stride, uint64_t n,
+                                       uint32_t* __restrict__ cells) {
+  const float inv = (float)kOccCells / 1000.0f;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t cx = (uint32_t)cell_of(px[i * stride], inv, kOccCells - 1);
+    const uint32_t cy = (uint32_t)cell_of(py[i * stride], inv, kOccCells - 1);
+    const uint32_t cz = (uint32_t)cell_of(pz[i * stride], inv, kOccCells - 1);
+    atomicAdd(cells + (cz * kOccCells + cy) * kOccCells + cx, 1u);
+  }
+}
+
+__global__ void occupancy_hist_kernel(const uint32_t* __restrict__ cells, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kOccBins];
+  if (threadIdx.x < kOccBins) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t ncell = (uint32_t)kOccCells * kOccCells * kOccCells;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    const uint32_t v = cells[c];
+    atomicAdd(&h[v ? 32 - __builtin_clz(v) : 0], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kOccBins && h[threadIdx.x]) atomicAdd(hist + threadIdx.x, h[threadIdx.x]);
+}
+
+}  // namespace
+
+hipError_t occupancy_histogram(const float* px, const float* py, const float* pz, uint32_t stride,
+                               uint64_t n, uint32_t* ws, uint32_t* hist, hipStream_t s) {
+  hipError_t e = fill_u32(ws, 0u, occupancy_workspace_words(), s);
+  if (e == hipSuccess) e = fill_u32(hist, 0u, kOccBins, s);
+  if (e != hipSuccess) return e;
+  if (n) {
+    uint64_t blocks = ceil_div(n, 256);
+    if (blocks > 16384) blocks = 16384;
+    occupancy_count_kernel<<<(uint32_t)blocks, 256, 0, s>>>(px, py, pz, stride, n, ws);
+  }
+  occupancy_hist_kernel<<<256, 256, 0, s>>>(ws, hist);
+  return hipGetLastError();
+}
+
 }  // namespace sjcs

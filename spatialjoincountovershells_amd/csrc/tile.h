@@ -21,6 +21,7 @@ struct TileView {
   const uint32_t* start;
   const uint32_t* len;
   uint32_t n;
+  uint32_t max_len;  // longest tile: sizes the LDS histograms of a wavefront
 };
 
 // Halos per tile the count can take for nrads shells (its LDS histograms
@@ -40,7 +41,8 @@ hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh
                        const TileView& tv, uint32_t* counts, bool accumulate,
                        unsigned long long* inbox, uint32_t* work, hipStream_t s);
 
-// dst[i] = src[perm[i]] for the four halo arrays (x, y, z, row).
+// dst[i] = src[perm[i]] for the halo arrays x, y, z and, when drow is not
+// null, row.
 hipError_t permute_halos(const float* sx, const float* sy, const float* sz, const uint32_t* srow,
                          float* dx, float* dy, float* dz, uint32_t* drow, const uint32_t* perm,
                          uint64_t n, hipStream_t s);

@@ -26,6 +26,8 @@
 // from a queue until it is empty.
 #include "tile.h"
 
+#include <type_traits>
+
 #include "pairs.h"
 #include "primitives.h"
 
@@ -104,8 +106,10 @@ __global__ void tile_cost_kernel(IndexView ix, HaloView hv, Shells sh, TileView 
   if (parts > kMaxParts) parts = kMaxParts;
   item[t] = (uint32_t)parts;
   if (zero_multi && parts > 1) {
-    uint32_t* dst = counts + (uint64_t)h0 * sh.nrads;
-    for (uint32_t i = 0; i < nt * (uint32_t)sh.nrads; ++i) dst[i] = 0u;
+    for (uint32_t i = 0; i < nt; ++i) {
+      uint32_t* dst = counts + (uint64_t)(hv.slot ? hv.slot[h0 + i] : h0 + i) * sh.nrads;
+      for (int k = 0; k < sh.nrads; ++k) dst[k] = 0u;
+    }
   }
 }
 
@@ -296,33 +300,36 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
           const uint32_t idx = rows[j] + (q - rows[64 + j]);
           const uint32_t boff = inq ? idx * 4u : 0xfffffffcu;  // out of range: loads 0
           const float px = buf_load(rx, boff), py = buf_load(ry, boff), pz = buf_load(rz, boff);
-          // the valid halos, kHaloBatch at a time: their LDS reads go out
-          // together, then the distances, then the shell-table reads, then the
-          // bin increments
-          for (int hb = 0; hb < nv; hb += kHaloBatch) {
-            int hh[kHaloBatch];
-            float4 c[kHaloBatch];
+          // the valid halos, NB at a time (8 while at least 8 are left, then
+          // 4, 2, 1: nv is the same on every lane, and a tile of copies of
+          // one position has a single valid halo): their LDS reads go out
+          // together, then the distances, then the shell-table reads, then
+          // the bin increments
+          auto batch = [&](auto nbc, int hb) {
+            constexpr int NB = decltype(nbc)::value;
+            int hh[NB];
+            float4 c[NB];
 #pragma unroll
-            for (int u = 0; u < kHaloBatch; ++u) hh[u] = (int)hl[min(hb + u, nv - 1)];
+            for (int u = 0; u < NB; ++u) hh[u] = (int)hl[hb + u];
 #pragma unroll
-            for (int u = 0; u < kHaloBatch; ++u)
+            for (int u = 0; u < NB; ++u)
               c[u] = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords);
             if constexpr (kInbox) {
 #pragma unroll
-              for (int u = 0; u < kHaloBatch; ++u) {
+              for (int u = 0; u < NB; ++u) {
                 const float4 b0 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 4);
                 const float4 b1 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 8);
-                const bool in = inq && (hb + u < nv) && !(px < b0.x) && !(b0.y < px) &&
-                                !(py < b0.z) && !(b0.w < py) && !(pz < b1.x) && !(b1.y < pz);
+                const bool in = inq && !(px < b0.x) && !(b0.y < px) && !(py < b0.z) &&
+                                !(b0.w < py) && !(pz < b1.x) && !(b1.y < pz);
                 inbox += in ? __builtin_bit_cast(uint32_t, b1.z) : 0u;
               }
             } else {
-              float dist[kHaloBatch];
-              bool hit[kHaloBatch];
+              float dist[NB];
+              bool hit[NB];
 #pragma unroll
-              for (int u = 0; u < kHaloBatch; ++u) {
+              for (int u = 0; u < NB; ++u) {
                 dist[u] = dpps_distance(px, py, pz, c[u].x, c[u].y, c[u].z);
-                hit[u] = inq & (hb + u < nv) & (dist[u] < sh.r2max);
+                hit[u] = inq & (dist[u] < sh.r2max);
                 if (hit[u] && !(dist[u] < c[u].w)) {
                   const float4 b0 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 4);
                   const float2 b1 = *reinterpret_cast<const float2*>(hd + hh[u] * kHaloWords + 8);
@@ -330,14 +337,14 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
                            !(pz < b1.x) && !(b1.y < pz);
                 }
               }
-              ShellEntry e[kHaloBatch];
+              ShellEntry e[NB];
 #pragma unroll
-              for (int u = 0; u < kHaloBatch; ++u) {
+              for (int u = 0; u < NB; ++u) {
                 const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
                 e[u] = g_lut[hit[u] ? max(b, 0) : 0];
               }
 #pragma unroll
-              for (int u = 0; u < kHaloBatch; ++u) {
+              for (int u = 0; u < NB; ++u) {
                 if (hit[u]) {
                   int o = (int)e[u].off + (dist[u] < e[u].t ? 0 : 4);
                   if constexpr (kShell == kShellTable)
@@ -346,7 +353,19 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
                 }
               }
             }
+          };
+          int hb = 0;
+          for (; hb + kHaloBatch <= nv; hb += kHaloBatch)
+            batch(std::integral_constant<int, kHaloBatch>{}, hb);
+          if (nv - hb >= 4) {
+            batch(std::integral_constant<int, 4>{}, hb);
+            hb += 4;
           }
+          if (nv - hb >= 2) {
+            batch(std::integral_constant<int, 2>{}, hb);
+            hb += 2;
+          }
+          if (nv - hb >= 1) batch(std::integral_constant<int, 1>{}, hb);
         }
         wave_sync();  // rows[] is rewritten by the next batch
       }
@@ -354,7 +373,7 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
     if (!kInbox) {
       wave_sync();
       for (int h = 0; h < nt; ++h) {
-        uint32_t* dst = counts + (uint64_t)(h0 + h) * K;
+        uint32_t* dst = counts + (uint64_t)(hv.slot ? hv.slot[h0 + h] : h0 + h) * K;
         const uint32_t* src = bins + s_rep[h] * K;
         for (int k = lane; k < K; k += 64) {
           const uint32_t v = src[k];
@@ -390,7 +409,7 @@ __global__ void permute_halos_kernel(const float* __restrict__ sx, const float* 
     dx[i] = sx[p];
     dy[i] = sy[p];
     dz[i] = sz[p];
-    drow[i] = srow[p];
+    if (drow) drow[i] = srow[p];
   }
 }
 
@@ -422,8 +441,12 @@ hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh
   hipError_t e = exclusive_sum_u32(item, item, (uint64_t)tv.n + 1, scan_ws, s);
   if (e == hipSuccess) e = fill_u32(queue, 0u, 1, s);
   if (e != hipSuccess) return e;
+  // histograms for the longest tile only: shorter tiles leave LDS for more
+  // wavefronts per CU
+  if (tv.max_len < 1 || tv.max_len > (uint32_t)tile_cap(sh.nrads)) return hipErrorInvalidValue;
   const size_t kpad = (size_t)((sh.nrads + 3) & ~3);
-  const size_t lds = 4 * (kpad + (size_t)kWaveWords);
+  const size_t bin_words = ((size_t)tv.max_len * (size_t)sh.nrads + 3) & ~(size_t)3;
+  const size_t lds = 4 * (kpad + (size_t)kWaveWords - kTileBinWords + bin_words);
   const int acc = accumulate ? 1 : 0;
   if (sh.one_step) {
     if (measure)

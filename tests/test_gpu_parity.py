@@ -570,6 +570,95 @@ def test_halo_tiles(eng, spec):
         eng.set_option("tiles", 1)
 
 
+@pytest.mark.parametrize("level,halos", [(1, 0), (2, 0), (3, 8), (4, 2), (6, 0), (3, 64)])
+def test_tile_levels(eng, level, halos):
+    """Tiles of neighbouring halos (halos in one Morton cell of 2^level grid
+    cells share their candidates), cut to at most `halos` per tile: counts
+    and W equal the oracle's, on clustered particles with halos at clump
+    centres, in the field and at the box corner."""
+    rng = np.random.default_rng(30 + level)
+    p = rng.random((60_000, 3), dtype=np.float32) * np.float32(1000.0)
+    clump = (np.array([1.0, 999.0, 2.0], np.float32) +
+             rng.normal(0, 1.5, (20_000, 3))).astype(np.float32) % np.float32(1000.0)
+    p = np.concatenate([p, clump]).astype(np.float32)
+    h = tiled_halo_set(rng)
+    near = (np.array([500.0, 500.0, 500.0], np.float32) +
+            rng.uniform(-12, 12, (300, 3))).astype(np.float32)
+    h = np.concatenate([h, near]).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.01, 20.0, 64)
+    want, w = oracle_chunks([p], h, r2)
+    try:
+        eng.set_option("tile_level", level)
+        eng.set_option("tile_halos", halos)
+        got = run_engine(eng, [p], h, r2)
+        assert_counts_equal(got, want, h)
+        assert eng.last_chunk_pair_tests() == w
+    finally:
+        eng.set_option("tile_level", 0)
+        eng.set_option("tile_halos", 0)
+
+
+def test_tile_level_changes_keep_counts(eng):
+    """Count rows do not depend on how halos are tiled: the tile level may
+    change between particle files while counts accumulate."""
+    rng = np.random.default_rng(36)
+    p, h = gen_uniform(37, 120_000, 4_000)
+    h[:400] = (np.float32(300.0) + rng.uniform(-6, 6, (400, 3))).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.01, 8.0, 16)
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    try:
+        for i, level in enumerate((0, 3, 1, 4)):
+            eng.set_option("tile_level", level)
+            eng.add_particles(p[i * 30_000:(i + 1) * 30_000])
+        got = eng.counts()
+    finally:
+        eng.set_option("tile_level", 0)
+    assert_counts_equal(got, oracle_chunks([p], h, r2)[0], h)
+
+
+def test_autotune_cache(built):
+    """A second autotune of the same workload takes the first one's setting
+    from the cache without timing; another workload (other shells) is
+    tuned afresh; with the cache off every call tunes."""
+    rng = np.random.default_rng(38)
+    p = rng.random((60_000, 3), dtype=np.float32) * np.float32(1000.0)
+    h = np.concatenate([np.repeat(np.array([[200.0, 300.0, 400.0]], np.float32), 9, axis=0),
+                        rng.random((800, 3), dtype=np.float32) * np.float32(1000.0)])
+    keys = ("cell_size_milli", "x_split", "lanes_per_halo", "segment", "tiles", "tile_level",
+            "tile_halos")
+    e = built.Engine(0)
+    try:
+        _, r2 = t.oracle_thresholds(0.02, 9.0, 24)
+        e.set_shells(r2)
+        e.set_halos(h)
+        d = e.device_array(p)
+        try:
+            ms1 = e.autotune(d, p.shape[0])
+            assert e.get_option("autotune_cache_hit") == 0
+            first = {k: e.get_option(k) for k in keys}
+            for k in ("cell_size_milli", "x_split", "tile_level"):
+                e.set_option(k, 0)
+            ms2 = e.autotune(d, p.shape[0])
+            assert e.get_option("autotune_cache_hit") == 1
+            assert ms2 == ms1
+            assert {k: e.get_option(k) for k in keys} == first
+            e.add_particles_device(d, p.shape[0])
+            assert_counts_equal(e.counts(), oracle_chunks([p], h, r2)[0], h)
+            _, r2b = t.oracle_thresholds(0.02, 9.5, 24)
+            e.set_shells(r2b)
+            e.autotune(d, p.shape[0])
+            assert e.get_option("autotune_cache_hit") == 0
+            e.set_option("autotune_cache", 0)
+            e.set_shells(r2)
+            e.autotune(d, p.shape[0])
+            assert e.get_option("autotune_cache_hit") == 0
+        finally:
+            e.device_free(d)
+    finally:
+        e.close()
+
+
 def test_autotune_keeps_counts(built):
     """The autotuner picks settings from timed runs; counts are unchanged by
     it and equal the oracle afterwards, on clustered data with tiles."""
