@@ -204,11 +204,14 @@ def wall_legs(parts, halos, radius, threads, with_reference):
                 *flag, "--halo-file", hf, "--radius", f"{mn}:{mx}:{k}",
                 "--output-file", os.devnull]
         res = {}
-        legs = [("sjcs_rcount", CLI_BIN)]
+        legs = [("sjcs_rcount", CLI_BIN, [])]
+        if len(parts) > 1:
+            # files held on the device ahead of the count (the reference's
+            # background index builds)
+            legs.append(("sjcs_rcount --backlog 2", CLI_BIN, ["--backlog", "2"]))
         if with_reference and os.path.exists(REF_BIN):
-            legs.append(("reference astr_rcount", REF_BIN))
-        for name, exe in legs:
-            extra = ["--backlog", "0"] if exe == REF_BIN else []
+            legs.append(("reference astr_rcount", REF_BIN, ["--backlog", "0"]))
+        for name, exe, extra in legs:
             t0 = time.time()
             out = subprocess.run([exe] + extra + args, capture_output=True, text=True, timeout=900)
             wall = time.time() - t0

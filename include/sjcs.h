@@ -312,7 +312,9 @@ int sjcs_index_create(sjcs_engine *e, uint64_t ndata, sjcs_index **out);
 int sjcs_index_insert(sjcs_index *ix, const float *xyz, uint64_t n);
 int sjcs_index_build(sjcs_index *ix);
 /* counts[h*nrads + k] += shell-k count of centre h, for n centres given as
- * in sjcs_set_halos.  Uses its own shells; leaves the engine's untouched. */
+ * in sjcs_set_halos.  Uses its own shells; leaves the engine's untouched.
+ * Centres are tiled as the engine's "tiles" / "tile_level" / "tile_halos"
+ * options say (centres at one spot share their candidate particles). */
 int sjcs_index_count(sjcs_index *ix, const float *centrals, size_t stride_bytes,
                      uint64_t n, const float *radiuses_sq, int nrads,
                      uint64_t *counts);

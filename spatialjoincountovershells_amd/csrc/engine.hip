@@ -493,15 +493,37 @@ int apply_tiles(sjcs_engine* e, HaloState& hs, int level) {
   const std::vector<uint32_t>& keys = hs.hkeys;
   std::vector<uint32_t> rs, rl;
   uint64_t ntiled = 0;
+  // runs of one Morton key first (halos at the same spot are evaluated once
+  // per tile); at a level above 0, runs of the coarser cell among the halos
+  // left between them
   for (uint64_t i = 0; i < n;) {
     uint64_t j = i + 1;
-    while (j < n && (keys[j] >> shift) == (keys[i] >> shift)) ++j;
-    if (j - i >= minrun) {
+    while (j < n && keys[j] == keys[i]) ++j;
+    if (j - i >= kTileMin) {
       rs.push_back((uint32_t)i);
       rl.push_back((uint32_t)(j - i));
       ntiled += j - i;
+      i = j;
+      continue;
     }
-    i = j;
+    if (level == 0) {
+      i = j;
+      continue;
+    }
+    // a run of the level's cell, stopping before the next run of one key
+    uint64_t k = i;
+    while (k < n && (keys[k] >> shift) == (keys[i] >> shift)) {
+      uint64_t m = k + 1;
+      while (m < n && keys[m] == keys[k]) ++m;
+      if (m - k >= kTileMin) break;
+      k = m;
+    }
+    if (k - i >= minrun) {
+      rs.push_back((uint32_t)i);
+      rl.push_back((uint32_t)(k - i));
+      ntiled += k - i;
+    }
+    i = k > i ? k : j;
   }
   if (!ntiled) return SJCS_OK;
   std::vector<uint32_t> perm;
@@ -788,12 +810,11 @@ int fold_counts(sjcs_engine* e) {
 // Tiles of the current halos for the current shells (host runs cut into
 // pieces of at most tile_cap halos, uploaded once); false when the count
 // runs without tiles.
-int ensure_tiles(sjcs_engine* e, bool* use) {
+int ensure_tiles(sjcs_engine* e, HaloState& hs, const Shells& sh, bool* use) {
   *use = false;
-  HaloState& hs = e->halos;
-  int cap = tile_cap(e->shells.nrads);
+  int cap = tile_cap(sh.nrads);
   if (e->tile_halos > 0 && e->tile_halos < cap) cap = (int)e->tile_halos;
-  if (!e->tiles || !e->shells.view.sorted || hs.n_tiled == 0 || cap < 2) return SJCS_OK;
+  if (!e->tiles || !sh.sorted || hs.n_tiled == 0 || cap < 2) return SJCS_OK;
   *use = true;
   if (hs.tcap == cap) return SJCS_OK;
   std::vector<uint32_t> ts, tl;
@@ -823,28 +844,28 @@ int ensure_tiles(sjcs_engine* e, bool* use) {
 
 // Launches the count of the current halos against index `ix` on stream st:
 // tiles (when `use_tiles`) and then the remaining halos one by one.
-hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, bool use_tiles, uint32_t* counts,
-                       bool accumulate, unsigned long long* inbox, hipStream_t st) {
-  HaloView hv = halo_view(e->halos);
-  const Shells& sh = e->shells.view;
+hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, const HaloState& hs, const Shells& sh,
+                       bool use_tiles, uint32_t* counts, bool accumulate,
+                       unsigned long long* inbox, hipStream_t st) {
+  HaloView hv = halo_view(hs);
+  const int lanes_n = lanes(e, ix.n, sh.rbox);
   if (!use_tiles)
-    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e, ix.n), seg_len(e), st);
+    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st);
   TileView tv;
-  tv.start = e->halos.tstart.as<uint32_t>();
-  tv.len = e->halos.tlen.as<uint32_t>();
-  tv.n = e->halos.ntiles;
-  tv.max_len = e->halos.tmax;
-  hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox,
-                             e->halos.twork.as<uint32_t>(), st);
+  tv.start = hs.tstart.as<uint32_t>();
+  tv.len = hs.tlen.as<uint32_t>();
+  tv.n = hs.ntiles;
+  tv.max_len = hs.tmax;
+  hipError_t h = count_tiles(ix, hv, sh, tv, counts, accumulate, inbox, hs.twork.as<uint32_t>(), st);
   if (h != hipSuccess) return h;
   // the other halos, after the tiled runs; slot[] names their count rows
-  const uint64_t nt = e->halos.n_tiled;
+  const uint64_t nt = hs.n_tiled;
   hv.x += nt;
   hv.y += nt;
   hv.z += nt;
   hv.slot += nt;
   hv.n -= nt;
-  return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes(e, ix.n), seg_len(e), st);
+  return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st);
 }
 
 int piece_done(sjcs_engine* e, uint64_t n);
@@ -874,7 +895,7 @@ int graph_piece(sjcs_engine* e, const ParticleSource& src, uint64_t n, const Gri
   const int sl = e->slot;
   bool grew = false, use_tiles = false;
   int rc = reserve_index(e, e->ibs[sl], n, g, &grew);
-  if (rc == SJCS_OK) rc = ensure_tiles(e, &use_tiles);
+  if (rc == SJCS_OK) rc = ensure_tiles(e, e->halos, e->shells.view, &use_tiles);
   if (rc != SJCS_OK) return rc;
   HIPCALL(e, hipStreamWaitEvent(e->stream, e->uploaded[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
@@ -892,7 +913,8 @@ int graph_piece(sjcs_engine* e, const ParticleSource& src, uint64_t n, const Gri
     int rq = enqueue_index(e, e->ibs[sl], src, n, g, e->stream);
     hipError_t hc = hipSuccess;
     if (rq == SJCS_OK)
-      hc = count_chunk(e, e->ibs[sl].view(), use_tiles, e->counts.as<uint32_t>(), !e->counts_fresh,
+      hc = count_chunk(e, e->ibs[sl].view(), e->halos, e->shells.view, use_tiles,
+                       e->counts.as<uint32_t>(), !e->counts_fresh,
                        nullptr, e->stream);
     const hipError_t he = hipStreamEndCapture(e->stream, &graph);
     if (rq != SJCS_OK) {
@@ -955,7 +977,7 @@ int add_piece(sjcs_engine* e, ParticleSource src, uint64_t n) {
   }
   if (e->graph) return graph_piece(e, src, n, g);
   bool use_tiles = false;
-  int rt = ensure_tiles(e, &use_tiles);
+  int rt = ensure_tiles(e, e->halos, e->shells.view, &use_tiles);
   if (rt != SJCS_OK) return rt;
   HIPCALL(e, hipStreamWaitEvent(e->bstream, e->counted[sl], 0), "hipStreamWaitEvent");
   if (!e->overlap && e->last >= 0)
@@ -971,7 +993,8 @@ int add_piece(sjcs_engine* e, ParticleSource src, uint64_t n) {
   HIPCALL(e, hipEventRecord(e->built[sl], e->bstream), "hipEventRecord");
   HIPCALL(e, hipStreamWaitEvent(e->stream, e->built[sl], 0), "hipStreamWaitEvent");
   HIPCALL(e, hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
-  HIPCALL(e, count_chunk(e, e->ibs[sl].view(), use_tiles, e->counts.as<uint32_t>(),
+  HIPCALL(e, count_chunk(e, e->ibs[sl].view(), e->halos, e->shells.view, use_tiles,
+                         e->counts.as<uint32_t>(),
                          !e->counts_fresh, nullptr, e->stream),
           "count kernel");
   HIPCALL(e, hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
@@ -1329,9 +1352,10 @@ int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
   if (!e->timed) return SJCS_OK;
   HIPCALL(e, fill_u64(e->inbox.as<uint64_t>(), 0ull, 1, e->stream), "fill");
   bool use_tiles = false;
-  rc = ensure_tiles(e, &use_tiles);
+  rc = ensure_tiles(e, e->halos, e->shells.view, &use_tiles);
   if (rc != SJCS_OK) return rc;
-  HIPCALL(e, count_chunk(e, e->ibs[e->last].view(), use_tiles, nullptr, true,
+  HIPCALL(e, count_chunk(e, e->ibs[e->last].view(), e->halos, e->shells.view, use_tiles,
+                         nullptr, true,
                          e->inbox.as<unsigned long long>(), e->stream),
           "pair-test count");
   HIPCALL(e, hipMemcpyAsync(pair_tests, e->inbox.p, 8, hipMemcpyDeviceToHost, e->stream),
@@ -1464,13 +1488,14 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
     }
     const Grid g = choose_grid(n, rbox, e->cell_size_milli, e->x_split);
     bool use_tiles = false;
-    int r = ensure_tiles(e, &use_tiles);
+    int r = ensure_tiles(e, e->halos, e->shells.view, &use_tiles);
     if (r != SJCS_OK) return r;
     for (int rep = 0; rep < 2; ++rep) {
       HIPCALL(e, hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
       r = build_index(e, e->ibs[0], packed(d_xyz), n, g, e->stream);
       if (r != SJCS_OK) return r;
-      HIPCALL(e, count_chunk(e, e->ibs[0].view(), use_tiles, trial_counts.as<uint32_t>(), false,
+      HIPCALL(e, count_chunk(e, e->ibs[0].view(), e->halos, e->shells.view, use_tiles,
+                             trial_counts.as<uint32_t>(), false,
                              nullptr, e->stream),
               "autotune count");
       HIPCALL(e, hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
@@ -1800,15 +1825,20 @@ int sjcs_index_count(sjcs_index* ix, const float* centrals, size_t stride_bytes,
   if (rc == SJCS_OK && !(sh.view.rbox < 499.0f) && ix->n > 0xffffffffull / 8)
     rc = fail(e, SJCS_ERR_INVALID, "index too large for this radius (counts could pass 2^32)");
   if (rc == SJCS_OK) rc = upload_points(e, pts, centrals, stride_bytes, n);
-  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), 12, n, false, false);
+  // the query halos in Morton order, tiled as the engine's options say:
+  // halos at one spot (or, at a tile level above 0, near one another) share
+  // their candidates, as the chunk loop's halos do
+  if (rc == SJCS_OK) rc = sort_halos(e, hs, pts.as<float>(), 12, n, true, false);
+  bool use_tiles = false;
+  if (rc == SJCS_OK) rc = ensure_tiles(e, hs, sh.view, &use_tiles);
   const size_t nc = (size_t)n * (size_t)nrads;
   std::vector<uint64_t> host;
   if (rc == SJCS_OK) {
     hipError_t h = ensure(dcounts, nc * sizeof(uint32_t));
     if (h == hipSuccess) h = ensure(ucounts, nc * sizeof(uint64_t));
     if (h == hipSuccess)
-      h = count_halos(ix->ib.view(), halo_view(hs), sh.view, dcounts.as<uint32_t>(), false,
-                      nullptr, lanes(e, ix->n, sh.view.rbox), seg_len(e), e->stream);
+      h = count_chunk(e, ix->ib.view(), hs, sh.view, use_tiles, dcounts.as<uint32_t>(), false,
+                      nullptr, e->stream);
     if (h == hipSuccess)
       h = readout_counts(dcounts.as<uint32_t>(), nullptr, hs.inv.as<uint32_t>(),
                          ucounts.as<uint64_t>(), n, nrads, e->stream);

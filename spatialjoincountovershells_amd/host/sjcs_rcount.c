@@ -5,8 +5,11 @@
  * Same argv, same stdout lines, same stderr messages and exit status for
  * bad options, same text result.  What runs differs: the halos, shell
  * thresholds and u64 counts live on the GPU (include/sjcs.h); each particle
- * file is indexed (cell sort) and counted there, and the next file is read
- * from disk while the GPU works on the current one.
+ * file is indexed (cell sort) and counted there.  --backlog N keeps up to N
+ * particle files on the device ahead of the one being counted (the
+ * reference's background index builds, main.c:266-350; "index n" and
+ * "index getting time" lines as it prints them, in file order); with
+ * --backlog 0 a file is read after the previous one is counted.
  *
  * Host side, in order of the wall time it saves:
  *  - the device is set up (runtime start, engine, pinned staging) on a
@@ -298,7 +301,8 @@ int main(int argc, char *argv[])
     struct timeval tv, tvtotal;
     char err[4096];
     float *rad, *r2;
-    chunk_buf bufs[2];
+    chunk_buf *bufs;
+    int nbufs, depth, ahead, next;
     setup_job setup;
     pthread_t setup_th;
     row_source src;
@@ -315,7 +319,20 @@ int main(int argc, char *argv[])
         sjcs_cli_print_usage(stderr, argv[0]);
         exit(EXIT_FAILURE);
     }
-    memset(bufs, 0, sizeof(bufs));
+    /* --backlog (main.c:266-350): the reference builds up to maxbacklogs
+     * indexes in the background while it searches.  Here that many particle
+     * files are held on the device ahead of the one being counted (and the
+     * next file's cell sort may then run while this one counts); 0 reads
+     * each file only after the previous one is counted, as
+     * noBacklogExecution builds each index after the previous search.  The
+     * first file is read while the halos are parsed either way. */
+    depth = cli.maxbacklogs > 0 ? cli.maxbacklogs : 0;
+    nbufs = depth + 1;
+    bufs = calloc((size_t)nbufs, sizeof(chunk_buf));
+    if (!bufs) {
+        fprintf(stderr, "ERROR: memory shortage\n");
+        exit(EXIT_FAILURE);
+    }
     memset(&setup, 0, sizeof(setup));
     hthreads = cli.nthreads < 256 ? cli.nthreads : 256;
     setup.device = dev_env ? atoi(dev_env) : 0;
@@ -368,7 +385,7 @@ int main(int argc, char *argv[])
         die_engine(e, "halos");
     trace("shells and halos on the device");
 
-    /* chunk loop (main.c:235-368): file i+1 is read while file i counts */
+    /* chunk loop (main.c:235-368) */
     nfiles = cli.particle_end - cli.particle_beg;
     TIMER_START(tvtotal);
     puts(cli.maxbacklogs == 0 ? "  -- with no backlog execution" : "  -- with backlog execution");
@@ -377,9 +394,30 @@ int main(int argc, char *argv[])
         fputs(setup.err, stderr);
         exit(EXIT_FAILURE);
     }
+    if (depth > 0 && sjcs_set_option(e, "overlap", 1) != SJCS_OK)
+        die_engine(e, "options");
+    next = 1;  /* the next file to read */
+    if (depth > 0) {
+        /* --first-setup a:b: a files read before the first count */
+        ahead = cli.first_nbacklogs < depth ? cli.first_nbacklogs : depth;
+        printf("index 0\n>>>> particles reading time: %ld milli seconds\n", bufs[0].read_ms);
+        for (; next < nfiles && next <= ahead; next++) {
+            printf("index %d\n", next);
+            read_chunk_or_die(e, setup.pin, argv[cli.particle_beg + next], &bufs[next % nbufs],
+                              cli.compressed, hthreads);
+            printf(">>>> particles reading time: %ld milli seconds\n", bufs[next % nbufs].read_ms);
+        }
+    }
     for (i = 0; i < nfiles; i++) {
-        chunk_buf *cur = &bufs[i & 1];
-        printf(">>>> particles reading time: %ld milli seconds\n", cur->read_ms);
+        chunk_buf *cur = &bufs[i % nbufs];
+        if (depth > 0) {
+            printf(">>>> index getting time: %ld milli seconds\n", 0L);
+        } else {
+            if (i > 0)
+                read_chunk_or_die(e, setup.pin, argv[cli.particle_beg + i], cur, cli.compressed,
+                                  hthreads);
+            printf(">>>> particles reading time: %ld milli seconds\n", cur->read_ms);
+        }
         if (cur->n == 0)
             fprintf(stderr, "%s(): No data loaded. Cannot make tree.\n",
                     "MakeTreeWithSameThreadPoolArraySTR");
@@ -389,10 +427,15 @@ int main(int argc, char *argv[])
         if (rc != SJCS_OK)
             die_engine(e, "particle chunk");
         trace("chunk queued");
-        if (i + 1 < nfiles) {
-            read_chunk_or_die(e, setup.pin, argv[cli.particle_beg + i + 1], &bufs[(i + 1) & 1],
+        /* up to `depth` files ahead while this one is sorted and counted;
+         * a file's buffer was last read by the chunk `depth` + 1 earlier,
+         * which is done */
+        for (; depth > 0 && next < nfiles && next <= i + depth; next++) {
+            printf("index %d\n", next);
+            read_chunk_or_die(e, setup.pin, argv[cli.particle_beg + next], &bufs[next % nbufs],
                               cli.compressed, hthreads);
-            trace("next file on the device");
+            printf(">>>> particles reading time: %ld milli seconds\n", bufs[next % nbufs].read_ms);
+            trace("file ahead on the device");
         }
         finish_chunk(e, cur->n, cli.compressed);
         trace("chunk counted");
@@ -419,9 +462,10 @@ int main(int argc, char *argv[])
 
     sjcs_host_free(src.rows[0]);
     sjcs_host_free(src.rows[1]);
-    for (i = 0; i < 2; i++)
+    for (i = 0; i < nbufs; i++)
         if (bufs[i].d)
             sjcs_device_free(e, bufs[i].d);
+    free(bufs);
     sjcs_host_free(setup.pin[0]);
     sjcs_host_free(setup.pin[1]);
     sjcs_engine_destroy(e);

@@ -158,6 +158,30 @@ def test_cli_result_file_many_batches(built, tmp_path, threads):
     assert len(got) == len(expect) and got == expect
 
 
+@pytest.mark.parametrize("backlog", [["--backlog", "1"], ["--backlog", "2", "--first-setup", "2:1"],
+                                     ["--backlog", "5", "--first-setup", "1:3"]])
+def test_cli_backlog_modes(built, backlog, tmp_path):
+    """--backlog N holds up to N particle files on the device ahead of the
+    one being counted: the result file is still the reference's, and stdout
+    has the backlog mode's lines (main.c:292-323): one "index n" per file,
+    an "index getting time" and a "searching time" per file."""
+    case = [c for c in CASES if c["name"] == "uni_3files"][0]
+    out = tmp_path / "result.txt"
+    args = [built.CLI_PATH, "--thread", "4", "--next-setup", "1", "--particle-files"] + \
+        case["particle_files"] + ["--halo-file", case["halo_file"], "--radius", case["radius"]] + \
+        backlog + ["--output-file", str(out)]
+    r = subprocess.run(args, cwd=t.GOLD, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert out.read_bytes() == t.read_expected_text(case["expected"])
+    n = len(case["particle_files"])
+    assert "  -- with backlog execution" in r.stdout
+    assert [ln for ln in r.stdout.splitlines() if ln.startswith("index ")] == \
+        [f"index {i}" for i in range(n)]
+    assert r.stdout.count(">>>> index getting time") == n
+    assert r.stdout.count(">>>> searching time") == n
+    assert r.stdout.count(">>>> particles reading time") == n
+
+
 def test_dpps_equivalence_on_the_gpu_host():
     """The DPPS probe (the FP32 distance as the reference's SSE4.1 dot
     product rounds it, against the oracle's plain expression) on the host
@@ -815,6 +839,37 @@ def test_index_api_matches_oracle(built, eng):
         want, _ = oracle_chunks([p], h, r2)
         assert_counts_equal(counts, want + np.uint64(7), h)
     ix.close()
+
+
+@pytest.mark.parametrize("level", [0, 2])
+def test_index_api_tiles(built, eng, level):
+    """The index count tiles its query halos like the chunk loop: halos at
+    one spot (runs of copies) and, at a tile level above 0, near one another
+    share candidates; counts equal the oracle and the untiled count."""
+    rng = np.random.default_rng(40 + level)
+    p = rng.random((100_000, 3), dtype=np.float32) * np.float32(1000.0)
+    clump = (np.float32(700.0) + rng.normal(0, 1.0, (30_000, 3))).astype(np.float32)
+    p = np.concatenate([p, clump]).astype(np.float32)
+    h = np.concatenate([np.repeat(np.array([[700.0, 700.0, 700.0]], np.float32), 30, axis=0),
+                        (np.float32(700.0) + rng.uniform(-4, 4, (40, 3))).astype(np.float32),
+                        tiled_halo_set(rng)]).astype(np.float32)
+    h = h[rng.permutation(h.shape[0])]
+    _, r2 = t.oracle_thresholds(0.01, 12.0, 32)
+    ix = built.ArraySTR(eng, p.shape[0])
+    try:
+        ix.insert(p)
+        ix.make_tree()
+        eng.set_option("tile_level", level)
+        got = ix.multi_range_count(h, r2)
+        eng.set_option("tiles", 0)
+        plain = ix.multi_range_count(h, r2)
+    finally:
+        eng.set_option("tiles", 1)
+        eng.set_option("tile_level", 0)
+        ix.close()
+    want, _ = oracle_chunks([p], h, r2)
+    assert_counts_equal(got, want, h)
+    assert np.array_equal(plain, got)
 
 
 # ------------------------------------------------ full size, by properties --
