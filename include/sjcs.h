@@ -219,6 +219,12 @@ int sjcs_last_chunk_pair_tests(sjcs_engine *e, uint64_t *pair_tests);
  *   "tile_halos"        most halos per tile, 2 to 64 (0 = as many as the
  *                       tile kernel's LDS histograms hold: 2048 / nrads);
  *                       smaller tiles need less LDS per wavefront
+ *   "stage_particles"   0 (default) = the count kernel reads candidates
+ *                       from HBM/L2; 256 to 8192 = each workgroup of 8 or 16
+ *                       lanes per halo first copies the particles of its
+ *                       halos' union box (at most this many) into LDS and
+ *                       reads the base images' candidates from there
+ *   "stage_halos"       halos per such workgroup (0 = 64; 4 to 1024)
  *   "autotune_cache"    1 (default) = sjcs_autotune reuses the setting it
  *                       found for the same workload in this process
  * sjcs_get_option also reads "autotune_cache_hit": 1 when the last
@@ -233,7 +239,8 @@ int sjcs_get_option(sjcs_engine *e, const char *key, int64_t *value);
  * chunk at d_xyz (device, n packed points) against the current halos and
  * shells under candidate settings -- lanes per halo, cell edge along y/z as
  * a fraction of r_box, x split, segment, then halo tiles on or off, tile
- * level and halos per tile, by coordinate descent -- and keeps the fastest
+ * level and halos per tile, then the particles staged in LDS per count
+ * workgroup and its halos, by coordinate descent -- and keeps the fastest
  * as the engine's options (sjcs_get_option reads them).  The setting is
  * remembered for the workload -- particle and halo counts, the thresholds,
  * and histograms of how particles and halos fill the box -- and reused

@@ -23,15 +23,26 @@ int count_group_for(int nrads, int requested);
 // Dynamic LDS of the count kernel for nrads shells at `group` lanes/halo.
 size_t count_lds_bytes(int nrads, int group);
 
+// Particle staging in the count kernel: a workgroup stages at most
+// stage_cap particles of the cell rows of its halos' union box, from at
+// most kStageRows rows.  count_stage_bytes: the LDS it adds.
+constexpr int kStageRows = kCountThreads;
+constexpr uint32_t kStageCapMax = 8192;
+constexpr size_t kMaxCountLds = 160 * 1024;
+size_t count_stage_bytes(uint32_t stage_cap);
+
 // counts[i*nrads + k] (= or +=) the shell-k count of sorted halo i against
 // the indexed particles (u32: the caller keeps a running total below 2^32,
 // one particle counting at most once per periodic image), `group` lanes per
 // halo (64: one halo per wavefront; 0: automatic), lanes walking `seg` (4 or
 // 8) particles of a row each.  With `inbox` not null, nothing is counted and
-// inbox receives += the number of in-box pair tests.  No allocation.
+// inbox receives += the number of in-box pair tests.  stage_cap > 0: the
+// count pass of 8 or 16 lanes per halo stages particles in LDS (at most
+// stage_cap per workgroup of stage_halos halos, 0 = 64).  No allocation.
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
                        uint32_t* counts, bool accumulate, unsigned long long* inbox,
-                       int group, int seg, hipStream_t s);
+                       int group, int seg, hipStream_t s, uint32_t stage_cap = 0,
+                       uint32_t stage_halos = 0);
 
 // c64[i] = (store ? 0 : c64[i]) + c32[i] for i < total.
 hipError_t widen_counts(const uint32_t* c32, uint64_t* c64, uint64_t total, bool store,

@@ -96,10 +96,34 @@ __device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
   return v;
 }
 
-template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox>
+// Inclusive sum over a workgroup of kCountThreads, one value per thread
+// (s_tmp: kCountWaves words); *total receives the sum of all.
+__device__ __forceinline__ uint32_t block_incl_sum(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) s_tmp[wave] = v;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kCountWaves; ++w) {
+    const uint32_t t = s_tmp[w];
+    before += w < wave ? t : 0u;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return v + before;
+}
+
+template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox, bool kStage>
 __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     IndexView ix, HaloView hv, Shells sh, uint32_t* __restrict__ counts,
-    uint32_t halos_per_block, int accumulate, unsigned long long* __restrict__ inbox_total) {
+    uint32_t halos_per_block, int accumulate, unsigned long long* __restrict__ inbox_total,
+    uint32_t stage_cap) {
   constexpr int H = 64 / GROUP;
   // Sorted shells: thresholds are finite or +inf, so a live halo's boxes are
   // NaN-free and a counted pair needs dist < r2max; each row's x run is cut
@@ -153,6 +177,115 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
   const int GX = ix.grid.gx;
   const int gxm1 = GX - 1;
   uint64_t inbox = 0;
+
+  // Particle staging (kStage, count pass with ascending shells): the
+  // workgroup copies the particles of the union of its live halos' base
+  // boxes into LDS -- the union's cell rows, each row's x run with
+  // coalesced loads, one wavefront per row -- and its halos' base images
+  // read their candidates from there.  s_delta[r] maps a particle index of
+  // union row r to its LDS slot.  Images across the periodic boundary, and
+  // workgroups whose union has more than kStageRows rows or stage_cap
+  // particles, read from HBM/L2 as without staging.
+  // LDS after the histograms: [s_delta kStageRows][row start, length,
+  // slot: 3 x kStageRows][union bounds and scan: 32][x, y, z: 3 x
+  // (stage_cap + 8)]
+  uint32_t* s_delta = s_bins + kCountWaves * H * K;
+  float* s_px = reinterpret_cast<float*>(s_delta + 4 * kStageRows + 32);
+  float* s_py = s_px + stage_cap + 8;
+  float* s_pz = s_py + stage_cap + 8;
+  bool staged = false;
+  int ux0 = 0, uy0 = 0, uz0 = 0, unyu = 0;
+  if constexpr (kStage) {
+    uint32_t* s_rgs = s_delta + kStageRows;
+    uint32_t* s_rlen = s_rgs + kStageRows;
+    uint32_t* s_roff = s_rlen + kStageRows;
+    float* s_uni = reinterpret_cast<float*>(s_roff + kStageRows);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t i = h0 + threadIdx.x; i < h1; i += kCountThreads) {
+      const float c[3] = {hv.x[i], hv.y[i], hv.z[i]};
+      if (c[0] == c[0] && c[1] == c[1] && c[2] == c[2]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = fminf(lo[d], c[d] - rbox);
+          hi[d] = fmaxf(hi[d], c[d] + rbox);
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        lo[d] = fminf(lo[d], __shfl_xor(lo[d], off, 64));
+        hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
+      }
+    // the waves' bounds, then every thread combines them (s_uni: 4 x 6)
+    if (lane == 0)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        s_uni[wave * 6 + d] = lo[d];
+        s_uni[wave * 6 + 3 + d] = hi[d];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kCountWaves; ++w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = fminf(lo[d], s_uni[w * 6 + d]);
+        hi[d] = fmaxf(hi[d], s_uni[w * 6 + 3 + d]);
+      }
+    // union cell ranges; a member's cells lie inside them (cell_of is
+    // monotone); no live halo: nothing to stage
+    const bool any = lo[0] <= hi[0];
+    ux0 = cell_of(lo[0], invx, gxm1);
+    const int ux1 = cell_of(hi[0], invx, gxm1);
+    uy0 = cell_of(lo[1], inv, gm1);
+    const int uy1 = cell_of(hi[1], inv, gm1);
+    uz0 = cell_of(lo[2], inv, gm1);
+    const int uz1 = cell_of(hi[2], inv, gm1);
+    unyu = uy1 - uy0 + 1;
+    const int urows = unyu * (uz1 - uz0 + 1);
+    if (any && urows <= kStageRows) {
+      // each union row's x run: its first particle and length, then the
+      // runs' slots in LDS by a prefix sum over the rows
+      const int r = threadIdx.x;
+      uint32_t gs = 0, len = 0;
+      if (r < urows) {
+        const int zq = r / unyu;
+        const uint32_t kb = ((uint32_t)(uz0 + zq) * (uint32_t)G + (uint32_t)(uy0 + r - zq * unyu)) *
+                            (uint32_t)GX;
+        const uint32_t k0 = kb + (uint32_t)ux0, k1 = kb + (uint32_t)ux1;
+        gs = k0 ? ix.cell_end[k0 - 1] : 0u;
+        len = ix.cell_end[k1] - gs;
+      }
+      uint32_t total;
+      const uint32_t incl = block_incl_sum(len, reinterpret_cast<uint32_t*>(s_uni + 24), &total);
+      staged = total <= stage_cap;  // the same on every thread
+      if (staged && r < urows) {
+        s_rgs[r] = gs;
+        s_rlen[r] = len;
+        s_roff[r] = incl - len;
+        s_delta[r] = incl - len - gs;  // LDS slot of particle i of row r: i + s_delta[r]
+      }
+      __syncthreads();
+      if (staged) {
+        for (int rr = wave; rr < urows; rr += kCountWaves) {
+          const uint32_t g0 = s_rgs[rr], n0 = s_rlen[rr], o0 = s_roff[rr];
+          for (uint32_t i = lane; i < n0; i += 64) {
+            s_px[o0 + i] = ix.x[g0 + i];
+            s_py[o0 + i] = ix.y[g0 + i];
+            s_pz[o0 + i] = ix.z[g0 + i];
+          }
+        }
+        // a slot past the end of the last row reads up to SEG - 1 beyond it
+        if (threadIdx.x < 8) {
+          s_px[total + threadIdx.x] = 0.0f;
+          s_py[total + threadIdx.x] = 0.0f;
+          s_pz[total + threadIdx.x] = 0.0f;
+        }
+      }
+      __syncthreads();
+    }
+  }
 
   for (uint64_t hb = h0 + (uint64_t)wave * H; hb < h1; hb += (uint64_t)kCountWaves * H) {
     const uint64_t hs = hb + grp;
@@ -227,7 +360,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       for (int rb = 0; rb < max_rows; rb += GROUP) {
         // this lane's row of the batch: start, length, and its segments
         const int r = rb + gl;
-        uint32_t beg = 0, len = 0;
+        uint32_t beg = 0, len = 0, sdelta = 0;
         if (r < nrows) {
           // r / ny: a reciprocal estimate is off by at most one here
           int zq = (int)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)ny));
@@ -258,8 +391,11 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             const uint32_t k0 = kb + (uint32_t)xa, k1 = kb + (uint32_t)xb;
             beg = k0 ? ix.cell_end[k0 - 1] : 0u;
             len = ix.cell_end[k1] - beg;
+            // the base image's rows lie in the staged union
+            if (kStage && staged && img == 0) sdelta = s_delta[(zz - uz0) * unyu + (yy - uy0)];
           }
         }
+        const bool from_lds = kStage && staged && img == 0;
         const uint32_t nseg = (len + SEG - 1) / SEG;
         const uint32_t sincl = grp_scan_add<GROUP>(nseg, gl);
         const uint32_t sexcl = sincl - nseg;
@@ -284,15 +420,30 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
           const uint32_t q = c0 + (uint32_t)gl;
           const uint32_t off = (q - rsx) * SEG;  // segment start within the row
           const uint32_t slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
-          const uint32_t boff = (rbeg + off) * 4u;
-          // a slot past its group's segments loads whatever its offset
-          // names (bounds-checked by the buffer descriptor) and never uses it
           float px[SEG], py[SEG], pz[SEG];
+          if (from_lds) {
+            // the staged row's slots; a slot past its group's segments reads
+            // slot 0, and one past its row's end at most SEG - 1 beyond the
+            // last row (padded): values it never uses
+            const uint32_t rsd = grp_lane<GROUP>(sdelta, j);
+            const uint32_t li = q < stotal ? rbeg + off + rsd : 0u;
 #pragma unroll
-          for (int u = 0; u < SEG; ++u) {
-            px[u] = buf_load(rx, boff + 4u * u);
-            py[u] = buf_load(ry, boff + 4u * u);
-            pz[u] = buf_load(rz, boff + 4u * u);
+            for (int u = 0; u < SEG; ++u) {
+              px[u] = s_px[li + u];
+              py[u] = s_py[li + u];
+              pz[u] = s_pz[li + u];
+            }
+          } else {
+            // a slot past its group's segments loads whatever its offset
+            // names (bounds-checked by the buffer descriptor) and never
+            // uses it
+            const uint32_t boff = (rbeg + off) * 4u;
+#pragma unroll
+            for (int u = 0; u < SEG; ++u) {
+              px[u] = buf_load(rx, boff + 4u * u);
+              py[u] = buf_load(ry, boff + 4u * u);
+              pz[u] = buf_load(rz, boff + 4u * u);
+            }
           }
           if constexpr (kSphere) {
             // count pass, ascending shells: a counted pair has dist < r2max
@@ -380,38 +531,60 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
   }
 }
 
-template <int GROUP, int kShell, bool kLds, bool kInbox>
-void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh,
-                  uint32_t* counts, int accumulate, unsigned long long* inbox,
-                  uint32_t nblocks, size_t lds, int seg, hipStream_t s) {
-  if (seg >= 8)
-    count_group_kernel<GROUP, 8, kShell, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
-        ix, hv, sh, counts, kHalosPerBlock, accumulate, inbox);
+// Launch shape of the count kernel.
+struct CountLaunch {
+  uint32_t nblocks;
+  size_t lds;
+  int seg;
+  uint32_t halos_per_block;
+  uint32_t stage_cap;  // 0: no particle staging
+};
+
+template <int GROUP, int kShell, bool kLds, bool kInbox, bool kStage>
+void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh, uint32_t* counts,
+                  int accumulate, unsigned long long* inbox, const CountLaunch& L,
+                  hipStream_t s) {
+  if (L.seg >= 8)
+    count_group_kernel<GROUP, 8, kShell, kLds, kInbox, kStage>
+        <<<L.nblocks, kCountThreads, L.lds, s>>>(ix, hv, sh, counts, L.halos_per_block,
+                                                 accumulate, inbox, L.stage_cap);
   else
-    count_group_kernel<GROUP, 4, kShell, kLds, kInbox><<<nblocks, kCountThreads, lds, s>>>(
-        ix, hv, sh, counts, kHalosPerBlock, accumulate, inbox);
+    count_group_kernel<GROUP, 4, kShell, kLds, kInbox, kStage>
+        <<<L.nblocks, kCountThreads, L.lds, s>>>(ix, hv, sh, counts, L.halos_per_block,
+                                                 accumulate, inbox, L.stage_cap);
 }
 
 template <int GROUP, int kShell>
 void dispatch_flags(bool lds, bool in, const IndexView& ix, const HaloView& hv,
                     const Shells& sh, uint32_t* counts, int acc, unsigned long long* inbox,
-                    uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
-  if (lds && !in) launch_group<GROUP, kShell, true, false>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
-  else if (lds && in) launch_group<GROUP, kShell, true, true>(ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
-  else if (!in) launch_group<GROUP, kShell, false, false>(ix, hv, sh, counts, 1, inbox, nb, ldsb, seg, s);
-  else launch_group<GROUP, kShell, false, true>(ix, hv, sh, counts, 1, inbox, nb, ldsb, seg, s);
+                    const CountLaunch& L, hipStream_t s) {
+  if (lds && !in) {
+    if constexpr (kShell != kShellScan && GROUP != 64) {
+      if (L.stage_cap) {
+        launch_group<GROUP, kShell, true, false, true>(ix, hv, sh, counts, acc, inbox, L, s);
+        return;
+      }
+    }
+    launch_group<GROUP, kShell, true, false, false>(ix, hv, sh, counts, acc, inbox, L, s);
+  } else if (lds && in) {
+    launch_group<GROUP, kShell, true, true, false>(ix, hv, sh, counts, acc, inbox, L, s);
+  } else if (!in) {
+    launch_group<GROUP, kShell, false, false, false>(ix, hv, sh, counts, 1, inbox, L, s);
+  } else {
+    launch_group<GROUP, kShell, false, true, false>(ix, hv, sh, counts, 1, inbox, L, s);
+  }
 }
 
 template <int GROUP>
 void dispatch_group(int shell, bool lds, bool in, const IndexView& ix, const HaloView& hv,
                     const Shells& sh, uint32_t* counts, int acc, unsigned long long* inbox,
-                    uint32_t nb, size_t ldsb, int seg, hipStream_t s) {
+                    const CountLaunch& L, hipStream_t s) {
   if (shell == kShellOneStep)
-    dispatch_flags<GROUP, kShellOneStep>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_flags<GROUP, kShellOneStep>(lds, in, ix, hv, sh, counts, acc, inbox, L, s);
   else if (shell == kShellTable)
-    dispatch_flags<GROUP, kShellTable>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_flags<GROUP, kShellTable>(lds, in, ix, hv, sh, counts, acc, inbox, L, s);
   else
-    dispatch_flags<GROUP, kShellScan>(lds, in, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_flags<GROUP, kShellScan>(lds, in, ix, hv, sh, counts, acc, inbox, L, s);
 }
 
 // c64[i] (= or +=) c32[i]: the chunk counts folded into the u64 totals.
@@ -505,6 +678,10 @@ int count_group_for(int nrads, int requested) {
   return nrads <= 256 ? 16 : 64;
 }
 
+size_t count_stage_bytes(uint32_t stage_cap) {
+  return stage_cap ? 4 * (4 * (size_t)kStageRows + 32 + 3 * ((size_t)stage_cap + 8)) : 0;
+}
+
 size_t count_lds_bytes(int nrads, int group) {
   if (nrads > kMaxLdsShells) return kNoLdsBytes;
   const size_t kpad = (size_t)((nrads + 3) & ~3);
@@ -514,7 +691,8 @@ size_t count_lds_bytes(int nrads, int group) {
 
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
                        uint32_t* counts, bool accumulate, unsigned long long* inbox,
-                       int group, int seg, hipStream_t s) {
+                       int group, int seg, hipStream_t s, uint32_t stage_cap,
+                       uint32_t stage_halos) {
   if (hv.n == 0 || sh.nrads <= 0) return hipSuccess;
   const bool lds = sh.nrads <= kMaxLdsShells;
   const bool measure = inbox != nullptr;
@@ -526,20 +704,27 @@ hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh
   }
   if (ix.n == 0) return hipSuccess;
   if (ix.n > 0x3fffffffull) return hipErrorInvalidValue;  // 32-bit byte offsets
-  uint64_t blocks = ceil_div(hv.n, kHalosPerBlock);
+  const int g = lds ? count_group_for(sh.nrads, group) : 64;
+  const int shell = !sh.sorted ? kShellScan : sh.one_step ? kShellOneStep : kShellTable;
+  CountLaunch L;
+  L.seg = seg;
+  // staging: the count pass with ascending shells and LDS histograms, 8 or
+  // 16 lanes per halo, within the LDS a workgroup can have
+  L.stage_cap = (stage_cap && lds && !measure && shell != kShellScan && g != 64) ? stage_cap : 0u;
+  L.halos_per_block = L.stage_cap && stage_halos ? stage_halos : kHalosPerBlock;
+  L.lds = count_lds_bytes(sh.nrads, g) + count_stage_bytes(L.stage_cap);
+  if (L.lds > kMaxCountLds) return hipErrorInvalidValue;
+  uint64_t blocks = ceil_div(hv.n, L.halos_per_block);
   blocks = (blocks + 7) & ~7ull;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-  const int g = lds ? count_group_for(sh.nrads, group) : 64;
-  const size_t ldsb = count_lds_bytes(sh.nrads, g);
+  L.nblocks = (uint32_t)blocks;
   const int acc = accumulate ? 1 : 0;
-  const uint32_t nb = (uint32_t)blocks;
-  const int shell = !sh.sorted ? kShellScan : sh.one_step ? kShellOneStep : kShellTable;
   if (g == 16)
-    dispatch_group<16>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_group<16>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, L, s);
   else if (g == 8)
-    dispatch_group<8>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_group<8>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, L, s);
   else
-    dispatch_group<64>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, nb, ldsb, seg, s);
+    dispatch_group<64>(shell, lds, measure, ix, hv, sh, counts, acc, inbox, L, s);
   return hipGetLastError();
 }
 

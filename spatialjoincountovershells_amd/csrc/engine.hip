@@ -303,6 +303,8 @@ struct sjcs_engine {
   int64_t tiles = 1;       // 1: halo tiles share their candidates (ascending shells)
   int64_t tile_level = 0;  // halos tiled together: one Morton cell of 2^level grid cells
   int64_t tile_halos = 0;  // most halos per tile (0: as many as the LDS histograms hold)
+  int64_t stage_particles = 0;  // particles a count workgroup stages in LDS (0: none)
+  int64_t stage_halos = 0;      // halos per count workgroup when staging (0: 64)
   int64_t autotune_cache = 1;  // 1: sjcs_autotune reuses the setting found for a workload
   bool autotune_hit = false;   // the last sjcs_autotune took its setting from the cache
 
@@ -849,8 +851,9 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, const HaloState& hs,
                        unsigned long long* inbox, hipStream_t st) {
   HaloView hv = halo_view(hs);
   const int lanes_n = lanes(e, ix.n, sh.rbox);
+  const uint32_t scap = (uint32_t)e->stage_particles, shal = (uint32_t)e->stage_halos;
   if (!use_tiles)
-    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st);
+    return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st, scap, shal);
   TileView tv;
   tv.start = hs.tstart.as<uint32_t>();
   tv.len = hs.tlen.as<uint32_t>();
@@ -865,7 +868,7 @@ hipError_t count_chunk(sjcs_engine* e, const IndexView& ix, const HaloState& hs,
   hv.z += nt;
   hv.slot += nt;
   hv.n -= nt;
-  return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st);
+  return count_halos(ix, hv, sh, counts, accumulate, inbox, lanes_n, seg_len(e), st, scap, shal);
 }
 
 int piece_done(sjcs_engine* e, uint64_t n);
@@ -1376,6 +1379,7 @@ int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
               {"overlap", e->overlap},                 {"index_sort", e->index_sort},
               {"piece_max", e->piece_max},             {"tile_level", e->tile_level},
               {"tile_halos", e->tile_halos},           {"autotune_cache", e->autotune_cache},
+              {"stage_particles", e->stage_particles}, {"stage_halos", e->stage_halos},
               {"autotune_cache_hit", e->autotune_hit ? 1 : 0}};
   for (const auto& o : opts)
     if (strcmp(key, o.k) == 0) {
@@ -1390,7 +1394,7 @@ int sjcs_get_option(sjcs_engine* e, const char* key, int64_t* value) {
 namespace {
 
 struct TuneSetting {
-  int64_t cell, xs, lanes, seg, tiles, level, thalos;
+  int64_t cell, xs, lanes, seg, tiles, level, thalos, stage, shalos;
 };
 
 // Autotuned settings of the workloads seen by this process (all engines).
@@ -1438,6 +1442,8 @@ void apply_setting(sjcs_engine* e, const TuneSetting& st) {
   e->tiles = st.tiles;
   e->tile_level = st.level;
   e->tile_halos = st.thalos;
+  e->stage_particles = st.stage;
+  e->stage_halos = st.shalos;
   e->kernel = 0;
   e->halos.tcap = 0;  // tiles cut again for this setting
   ++e->epoch;
@@ -1459,8 +1465,9 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
   DevBuf trial_counts;
   HIPCALL(e, ensure(trial_counts, 4 * trial_words), "hipMalloc(autotune)");
   const float rbox = e->shells.view.rbox;
-  TuneSetting cur{e->cell_size_milli, e->x_split, e->lanes_per_halo, e->segment, e->tiles,
-                  e->tile_level, e->tile_halos};
+  TuneSetting cur{e->cell_size_milli, e->x_split,     e->lanes_per_halo,
+                  e->segment,         e->tiles,       e->tile_level,
+                  e->tile_halos,      e->stage_particles, e->stage_halos};
   if (e->kernel == 1) cur.lanes = 64;
   std::string key;
   if (e->autotune_cache) {
@@ -1568,6 +1575,21 @@ int sjcs_autotune(sjcs_engine* e, const float* d_xyz, uint64_t n, float* best_ms
           rc = try_set(st);
         }
   }
+  // particle staging in LDS (depth = particles per workgroup), then the
+  // halos per staging workgroup
+  for (int64_t sp : {0, 1024, 2048, 4096})
+    if (rc == SJCS_OK && sp != cur.stage) {
+      TuneSetting st = cur;
+      st.stage = sp;
+      rc = try_set(st);
+    }
+  if (cur.stage)
+    for (int64_t sh : {16, 32, 128, 0})
+      if (rc == SJCS_OK && sh != cur.shalos) {
+        TuneSetting st = cur;
+        st.shalos = sh;
+        rc = try_set(st);
+      }
   // keep the winner; the index buffers of slot 0 no longer hold the last chunk
   apply_setting(e, cur);
   if (rc == SJCS_OK && e->halos.level != (int)cur.level) rc = apply_tiles(e, e->halos, (int)cur.level);
@@ -1601,6 +1623,19 @@ int sjcs_set_option(sjcs_engine* e, const char* key, int64_t value) {
     e->tile_level = value;
     // count rows stay the Morton positions: counts in progress are kept
     if (e->have_halos && e->halos.level != (int)value) return apply_tiles(e, e->halos, (int)value);
+    return SJCS_OK;
+  }
+  if (strcmp(key, "stage_particles") == 0) {
+    if (value < 0 || (value > 0 && value < 256) || value > (int64_t)kStageCapMax)
+      return fail(e, SJCS_ERR_INVALID,
+                  "stage_particles: 0 or 256 to " + std::to_string(kStageCapMax));
+    e->stage_particles = value;
+    return SJCS_OK;
+  }
+  if (strcmp(key, "stage_halos") == 0) {
+    if (value < 0 || (value > 0 && value < 4) || value > 1024)
+      return fail(e, SJCS_ERR_INVALID, "stage_halos: 0 or 4 to 1024");
+    e->stage_halos = value;
     return SJCS_OK;
   }
   if (strcmp(key, "autotune_cache") == 0) {

@@ -130,13 +130,15 @@ hipError_t halo_keys(const float* pts, size_t stride_bytes, uint64_t n, uint32_t
 hipError_t has_key_runs(const uint32_t* keys, uint64_t n, uint32_t run, uint32_t* flag,
                         hipStream_t s);
 
-// A coarse description of how points fill the box: the points are counted
-// in kOccCells^3 cells (coordinates px[i*stride], py[i*stride], pz[i*stride],
-// clamped into the box), and hist[b] (kOccBins words) receives the number
-// of cells holding between 2^(b-1) and 2^b - 1 points (b = 0: empty cells).
-// ws holds occupancy_workspace_words() words.
+// A coarse description of how points fill the box: every step-th point
+// (at most kOccSample of them) is counted in kOccCells^3 cells (coordinates
+// px[i*stride], py[i*stride], pz[i*stride], clamped into the box), and
+// hist[b] (kOccBins words) receives the number of cells holding between
+// 2^(b-1) and 2^b - 1 of them (b = 0: empty cells).  ws holds
+// occupancy_workspace_words() words.
 constexpr int kOccCells = 64;
 constexpr int kOccBins = 33;
+constexpr uint64_t kOccSample = 1u << 22;
 inline uint64_t occupancy_workspace_words() {
   return (uint64_t)kOccCells * kOccCells * kOccCells;
 }

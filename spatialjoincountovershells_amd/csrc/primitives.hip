@@ -1108,12 +1108,14 @@ hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n, uint32_t* cel
 
 namespace {
 
+// Points i * step (i < m) counted into their cells.
 __global__ void occupancy_count_kernel(const float* __restrict__ px, const float* __restrict__ py,
-                                       const float* __restrict__ pz, uint32_t stride, uint64_t n,
+                                       const float* __restrict__ pz, uint64_t stride, uint64_t m,
                                        uint32_t* __restrict__ cells) {
   const float inv = (float)kOccCells / 1000.0f;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = j * stride;
     const uint32_t cx = (uint32_t)cell_of(px[i * stride], inv, kOccCells - 1);
     const uint32_t cy = (uint32_t)cell_of(py[i * stride], inv, kOccCells - 1);
     const uint32_t cz = (uint32_t)cell_of(pz[i * stride], inv, kOccCells - 1);
@@ -1142,9 +1144,14 @@ hipError_t occupancy_histogram(const float* px, const float* py, const float* pz
   if (e == hipSuccess) e = fill_u32(hist, 0u, kOccBins, s);
   if (e != hipSuccess) return e;
   if (n) {
-    uint64_t blocks = ceil_div(n, 256);
+    // every step-th point, at most kOccSample of them: a description, not
+    // a count (clustered points make the cell atomics contend)
+    const uint64_t step = ceil_div(n, kOccSample);
+    const uint64_t m = ceil_div(n, step);
+    uint64_t blocks = ceil_div(m, 256);
     if (blocks > 16384) blocks = 16384;
-    occupancy_count_kernel<<<(uint32_t)blocks, 256, 0, s>>>(px, py, pz, stride, n, ws);
+    occupancy_count_kernel<<<(uint32_t)blocks, 256, 0, s>>>(px, py, pz, (uint64_t)stride * step, m,
+                                                             ws);
   }
   occupancy_hist_kernel<<<256, 256, 0, s>>>(ws, hist);
   return hipGetLastError();

@@ -368,6 +368,32 @@ def test_kernel_shapes(eng, lanes, segment):
         eng.set_option("segment", 0)
 
 
+@pytest.mark.parametrize("lanes,segment,stage,halos", [
+    (16, 4, 2048, 0), (16, 8, 4096, 16), (8, 4, 1024, 32), (16, 4, 256, 64), (8, 8, 8192, 128)])
+def test_particle_staging(eng, lanes, segment, stage, halos):
+    """Particles staged in LDS per workgroup (the union of its halos' base
+    boxes): the counts equal the oracle's, with images across the periodic
+    boundary read from HBM, workgroups whose union does not fit (small
+    stage_particles) falling back, dense clumps and sparse regions."""
+    p, h = gen_uniform(60 + lanes + segment, 150_000, 6_000)
+    rng = np.random.default_rng(61)
+    clump = (np.float32(400.0) + rng.normal(0, 2.0, (40_000, 3))).astype(np.float32)
+    p = np.concatenate([p, clump]).astype(np.float32)
+    h[:300] = (np.float32(400.0) + rng.uniform(-3, 3, (300, 3))).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.01, 6.0, 24)
+    want, _ = oracle_chunks([p], h, r2)
+    try:
+        eng.set_option("lanes_per_halo", lanes)
+        eng.set_option("segment", segment)
+        eng.set_option("stage_particles", stage)
+        eng.set_option("stage_halos", halos)
+        got = run_engine(eng, [p], h, r2)
+    finally:
+        for k in ("lanes_per_halo", "segment", "stage_particles", "stage_halos"):
+            eng.set_option(k, 0)
+    assert_counts_equal(got, want, h)
+
+
 @pytest.mark.parametrize("index_sort", [0, 1])
 def test_clustered_particles(eng, index_sort):
     """Most particles in one tight clump (whole sort tiles of one cell, one
