@@ -31,16 +31,31 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xf, 0xf, false);
 }
 
-// Inclusive sum / max over the GROUP lanes of a group (values >= 0).  For
-// GROUP 8 a DPP row holds two groups, so a step that would reach into the
-// group below is masked.
+// DPP row_bcast:15 (lane 15 of each row to the next row: rows 1 and 3
+// enabled) and row_bcast:31 (lane 31 to rows 2 and 3); disabled rows read 0.
+__device__ __forceinline__ uint32_t row_bcast15(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+}
+__device__ __forceinline__ uint32_t row_bcast31(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+}
+
+// Inclusive sum / max over the GROUP lanes of a group (values >= 0), in
+// registers: DPP shifts within rows of 16 lanes, then (GROUP 64) the row
+// broadcasts carry each row's total into the rows above.  For GROUP 8 a DPP
+// row holds two groups, so a step that would reach into the group below is
+// masked.
 template <int GROUP>
 __device__ __forceinline__ uint32_t grp_scan_add(uint32_t v, int gl) {
-  if constexpr (GROUP == 16) {
+  if constexpr (GROUP == 16 || GROUP == 64) {
     v += row_shr<1>(v);
     v += row_shr<2>(v);
     v += row_shr<4>(v);
     v += row_shr<8>(v);
+    if constexpr (GROUP == 64) {
+      v += row_bcast15(v);
+      v += row_bcast31(v);
+    }
   } else if constexpr (GROUP == 8) {
     uint32_t t = row_shr<1>(v);
     v += gl >= 1 ? t : 0u;
@@ -60,11 +75,15 @@ __device__ __forceinline__ uint32_t grp_scan_add(uint32_t v, int gl) {
 
 template <int GROUP>
 __device__ __forceinline__ uint32_t grp_scan_max(uint32_t v, int gl) {
-  if constexpr (GROUP == 16) {
+  if constexpr (GROUP == 16 || GROUP == 64) {
     v = max(v, row_shr<1>(v));
     v = max(v, row_shr<2>(v));
     v = max(v, row_shr<4>(v));
     v = max(v, row_shr<8>(v));
+    if constexpr (GROUP == 64) {
+      v = max(v, row_bcast15(v));
+      v = max(v, row_bcast31(v));
+    }
   } else if constexpr (GROUP == 8) {
     uint32_t t = row_shr<1>(v);
     v = max(v, gl >= 1 ? t : 0u);
