@@ -1108,17 +1108,18 @@ hipError_t build_cell_end(const uint32_t* sorted_keys, uint64_t n, uint32_t* cel
 
 namespace {
 
-// Points i * step (i < m) counted into their cells.
+// Points j * step (j < m) counted into their cells; `stride` is the element
+// stride of one point times step, so point j * step is element j * stride.
 __global__ void occupancy_count_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                        const float* __restrict__ pz, uint64_t stride, uint64_t m,
                                        uint32_t* __restrict__ cells) {
   const float inv = (float)kOccCells / 1000.0f;
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
        j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = j * stride;
-    const uint32_t cx = (uint32_t)cell_of(px[i * stride], inv, kOccCells - 1);
-    const uint32_t cy = (uint32_t)cell_of(py[i * stride], inv, kOccCells - 1);
-    const uint32_t cz = (uint32_t)cell_of(pz[i * stride], inv, kOccCells - 1);
+    const uint64_t i = j * stride;  // element index: point j * step of the input
+    const uint32_t cx = (uint32_t)cell_of(px[i], inv, kOccCells - 1);
+    const uint32_t cy = (uint32_t)cell_of(py[i], inv, kOccCells - 1);
+    const uint32_t cz = (uint32_t)cell_of(pz[i], inv, kOccCells - 1);
     atomicAdd(cells + (cz * kOccCells + cy) * kOccCells + cx, 1u);
   }
 }
