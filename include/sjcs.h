@@ -340,6 +340,52 @@ void sjcs_index_destroy(sjcs_index *ix);
 int sjcs_index_count_noalloc(sjcs_index *ix, const float *d_centrals, uint64_t n,
                              const float *radiuses_sq, int nrads, uint64_t *d_counts);
 
+/* ---- R-tree: rt_rcount's index and count ----
+ * rt_rcount counts with a Guttman R-tree whose leaf walk differs from
+ * astr_rcount's: a leaf under an overlapping parent tests every entry's
+ * distance with no box test, and an entry at distance exactly 0 from the
+ * (moved) centre ends that leaf's scan -- neither it nor the entries after
+ * it in the leaf are counted (src/objects/RTree/MultiRangeCount.c:86-114).
+ * Which particles share a leaf depends on the insertion order, so the tree
+ * is built as the reference builds it, one point at a time on the host, and
+ * walked on the GPU:
+ *
+ *   reference                                         here
+ *   constructRTree           RTree.c:13-25            sjcs_rtree_create
+ *   insertDataRTree  x n     RTree.c:35-49,           sjcs_rtree_insert
+ *                            rt_rcount/main.c:711-715
+ *   threadJoin + multiRangeCountWithPriodicBoundRTree
+ *                            rt_rcount/main.c:735-764,
+ *                            RTree/MultiRangeCount.c:13-114
+ *                                                      sjcs_rtree_count
+ *   destructRTree            RTree.c:27-33            sjcs_rtree_destroy */
+typedef struct sjcs_rtree sjcs_rtree;
+
+/* e may be NULL: a host-only tree that can be built and inspected but not
+ * counted with. */
+int sjcs_rtree_create(sjcs_engine *e, sjcs_rtree **out);
+/* Inserts n points (packed FP32 xyz, host) in order; their ids continue
+ * from the points already inserted.  At most 2^29-1 points per tree.
+ * SJCS_ERR_INVALID when the reference's split is undefined for the
+ * coordinates (every enlargement difference NaN, where it reads outside its
+ * entries); the tree is unusable after a failed insert. */
+int sjcs_rtree_insert(sjcs_rtree *rt, const float *xyz, uint64_t n);
+/* Points, leaves, nodes and height (levels above the leaves). */
+int sjcs_rtree_shape(const sjcs_rtree *rt, uint64_t *npoints, uint64_t *nleaves,
+                     uint64_t *nnodes, int *height);
+/* The leaf grouping: ids[npoints] leaf by leaf (leaves depth first, children
+ * and entries in node order), leaf_len[nleaves].  No reference counterpart;
+ * for inspection and tests. */
+int sjcs_rtree_leaves(const sjcs_rtree *rt, uint64_t *ids, uint32_t *leaf_len);
+/* counts[h*nrads + k] += rt_rcount's shell-k count of centre h, for n
+ * centres given as in sjcs_set_halos, 1 to 1024 shells; on the engine's
+ * GPU (the tree is copied there on the first count after an insert). */
+int sjcs_rtree_count(sjcs_rtree *rt, const float *centrals, size_t stride_bytes, uint64_t n,
+                     const float *radiuses_sq, int nrads, uint64_t *counts);
+/* Message of the tree's last error (also sjcs_last_error of its engine). */
+const char *sjcs_rtree_error(const sjcs_rtree *rt);
+void sjcs_rtree_destroy(sjcs_rtree *rt);
+
 #ifdef __cplusplus
 }
 #endif

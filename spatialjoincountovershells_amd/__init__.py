@@ -11,6 +11,9 @@ The names follow the reference's interface for this path
 * :class:`ArraySTR` -- the index lifecycle ``constructArraySTR`` /
   ``insertDataArraySTR`` / ``makeTree*ArraySTR`` /
   ``multiRangeCountWithPriodicBoundArraySTR`` / ``destructArraySTR``.
+* :class:`RTree` -- ``rt_rcount``'s index: ``constructRTree`` /
+  ``insertDataRTree`` / ``multiRangeCountWithPriodicBoundRTree`` /
+  ``destructRTree``.
 * :func:`shell_thresholds` -- the CLI's ``--radius min:max:K`` to squared
   FP32 thresholds (include/support_functions.h:18-34, main.c:506-533).
 
@@ -52,6 +55,8 @@ EXPORTED = (
     "sjcs_decode_compressed_device",
     "sjcs_server_create", "sjcs_server_submit", "sjcs_server_test", "sjcs_server_wait",
     "sjcs_server_times", "sjcs_server_release", "sjcs_server_error", "sjcs_server_destroy",
+    "sjcs_rtree_create", "sjcs_rtree_insert", "sjcs_rtree_shape", "sjcs_rtree_leaves",
+    "sjcs_rtree_count", "sjcs_rtree_error", "sjcs_rtree_destroy",
 )
 
 
@@ -200,6 +205,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_add_compressed_particles_device": (i32, [vp, vp, u64,
                                                        ctypes.POINTER(GadgetDecode)]),
         "sjcs_decode_compressed_device": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode), vp]),
+        "sjcs_rtree_create": (i32, [vp, pp]),
+        "sjcs_rtree_insert": (i32, [vp, vp, u64]),
+        "sjcs_rtree_shape": (i32, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                   ctypes.POINTER(u64), ctypes.POINTER(i32)]),
+        "sjcs_rtree_leaves": (i32, [vp, vp, vp]),
+        "sjcs_rtree_count": (i32, [vp, vp, sz, u64, vp, i32, vp]),
+        "sjcs_rtree_error": (ctypes.c_char_p, [vp]),
+        "sjcs_rtree_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -414,6 +427,77 @@ class ArraySTR:
         if getattr(self, "_h", None):
             self._lib.sjcs_index_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class RTree:
+    """rt_rcount's index (src/objects/RTree/RTree.c, Build.c): a Guttman
+    R-tree built on the host by inserting points one at a time in order --
+    the order decides which particles share a leaf, and rt_rcount's count
+    depends on that -- and counted on the GPU with rt_rcount's leaf walk
+    (src/objects/RTree/MultiRangeCount.c:13-114).  ``engine`` None gives a
+    host-only tree: insert and inspect, no count."""
+
+    def __init__(self, engine: "Engine | None" = None):
+        self._e = engine
+        self._lib = engine._lib if engine is not None else load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.sjcs_rtree_create(engine._h if engine is not None else None,
+                                         ctypes.byref(h))
+        if rc != SJCS_OK:
+            raise SJCSError(rc, "sjcs_rtree_create")
+        self._h = h
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise SJCSError(rc, self._lib.sjcs_rtree_error(self._h).decode())
+        return rc
+
+    def insert(self, xyz):
+        """insertDataRTree for each point, in order (ids continue)."""
+        p = _points(xyz, "particles")
+        self._check(self._lib.sjcs_rtree_insert(self._h, _ptr(p), p.shape[0]))
+
+    def shape(self) -> dict:
+        n, nl, nn, ht = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        self._check(self._lib.sjcs_rtree_shape(self._h, ctypes.byref(n), ctypes.byref(nl),
+                                               ctypes.byref(nn), ctypes.byref(ht)))
+        return {"points": n.value, "leaves": nl.value, "nodes": nn.value, "height": ht.value}
+
+    def leaves(self):
+        """(ids leaf by leaf, leaf sizes), leaves depth first."""
+        sh = self.shape()
+        ids = np.zeros(max(sh["points"], 1), np.uint64)
+        lens = np.zeros(max(sh["leaves"], 1), np.uint32)
+        self._check(self._lib.sjcs_rtree_leaves(self._h, _ptr(ids), _ptr(lens)))
+        return ids[:sh["points"]], lens[:sh["leaves"]]
+
+    def multi_range_count(self, centrals, radiuses_sq, counts=None) -> np.ndarray:
+        """threadJoin + multiRangeCountWithPriodicBoundRTree over every
+        centre: counts [n][nrads] (uint64, added to when given)."""
+        c = _points(centrals, "centrals")
+        r2 = _shells(radiuses_sq)
+        if counts is None:
+            counts = np.zeros((c.shape[0], r2.size), dtype=np.uint64)
+        if counts.dtype != np.uint64 or not counts.flags.c_contiguous or \
+                counts.shape != (c.shape[0], r2.size):
+            raise ValueError("counts must be C-contiguous uint64 (n, nrads)")
+        self._check(self._lib.sjcs_rtree_count(self._h, _ptr(c), 12, c.shape[0], _ptr(r2),
+                                               r2.size, _ptr(counts)))
+        return counts
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sjcs_rtree_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def __del__(self):
         self.close()

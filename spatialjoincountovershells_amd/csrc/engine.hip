@@ -26,6 +26,7 @@
 #include "count.h"
 #include "gadget.h"
 #include "primitives.h"
+#include "rtree.h"
 #include "tile.h"
 
 using namespace sjcs;
@@ -782,7 +783,6 @@ int lanes(const sjcs_engine* e, uint64_t n, float rbox) {
   const double per_box = (double)n * side * side * side;
   return !(per_box <= 4096.0) ? 64 : 16;
 }
-int lanes(const sjcs_engine* e, uint64_t n) { return lanes(e, n, e->shells.view.rbox); }
 int seg_len(const sjcs_engine* e) { return e->segment ? (int)e->segment : 4; }
 
 // Most images of one halo a particle can lie in: 1 unless the query box is
@@ -2446,6 +2446,203 @@ void sjcs_server_destroy(sjcs_server* sv) {
     sl.release();
   }
   delete sv;
+}
+
+
+// ---- R-tree API (rt_rcount's index and count) ----
+
+}  // extern "C"
+
+struct sjcs_rtree {
+  sjcs_engine* e = nullptr;  // null: a host-only tree (build and inspect)
+  RtTree tree;
+  std::string err;
+  bool on_device = false;  // the device arrays hold the current tree
+  DevBuf meta, lo[3], hi[3], child;
+  void release() {
+    meta.release();
+    child.release();
+    for (int d = 0; d < 3; ++d) {
+      lo[d].release();
+      hi[d].release();
+    }
+    on_device = false;
+  }
+};
+
+namespace {
+
+int rt_fail(sjcs_rtree* rt, int code, const std::string& msg) {
+  rt->err = msg;
+  if (rt->e) rt->e->err = msg;
+  return code;
+}
+
+int rt_hip_fail(sjcs_rtree* rt, hipError_t h, const char* what) {
+  return rt_fail(rt, h == hipErrorOutOfMemory ? SJCS_ERR_NOMEM : SJCS_ERR_DEVICE,
+                 std::string(what) + ": " + hipGetErrorString(h));
+}
+
+template <class T>
+hipError_t upload_vec(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  hipError_t h = ensure(b, sizeof(T) * (v.size() ? v.size() : 1));
+  if (h == hipSuccess && !v.empty())
+    h = hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+  return h;
+}
+
+// The tree's node arrays onto the device (once per state of the tree).
+int rt_upload(sjcs_rtree* rt) {
+  if (rt->on_device) return SJCS_OK;
+  sjcs_engine* e = rt->e;
+  RtFlat f;
+  try {
+    rt->tree.flatten(&f);
+  } catch (const std::bad_alloc&) {
+    return rt_fail(rt, SJCS_ERR_NOMEM, "host memory for the R-tree layout");
+  }
+  if (f.inner != 0 && (f.meta[0] & kRtLeaf))
+    return rt_fail(rt, SJCS_ERR_STATE, "R-tree layout: a leaf above an inner node");
+  rt->release();
+  hipError_t h = upload_vec(rt->meta, f.meta, e->stream);
+  for (int d = 0; d < 3 && h == hipSuccess; ++d) {
+    h = upload_vec(rt->lo[d], f.lo[d], e->stream);
+    if (h == hipSuccess) h = upload_vec(rt->hi[d], f.hi[d], e->stream);
+  }
+  if (h == hipSuccess) h = upload_vec(rt->child, f.child, e->stream);
+  // the host vectors go out of scope here: wait for the copies
+  if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+  if (h != hipSuccess) {
+    rt->release();
+    return rt_hip_fail(rt, h, "R-tree upload");
+  }
+  rt->on_device = true;
+  return SJCS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sjcs_rtree_create(sjcs_engine* e, sjcs_rtree** out) {
+  if (!out) return SJCS_ERR_INVALID;
+  *out = nullptr;
+  sjcs_rtree* rt = nullptr;
+  try {
+    rt = new sjcs_rtree();
+  } catch (const std::bad_alloc&) {
+    return SJCS_ERR_NOMEM;
+  }
+  rt->e = e;
+  *out = rt;
+  return SJCS_OK;
+}
+
+int sjcs_rtree_insert(sjcs_rtree* rt, const float* xyz, uint64_t n) {
+  if (!rt) return SJCS_ERR_INVALID;
+  if (!xyz && n) return rt_fail(rt, SJCS_ERR_INVALID, "null points");
+  std::string why;
+  try {
+    if (!rt->tree.insert(xyz, n, &why)) return rt_fail(rt, SJCS_ERR_INVALID, why);
+  } catch (const std::bad_alloc&) {
+    return rt_fail(rt, SJCS_ERR_NOMEM, "host memory for the R-tree");
+  }
+  if (n) rt->on_device = false;
+  return SJCS_OK;
+}
+
+int sjcs_rtree_shape(const sjcs_rtree* rt, uint64_t* npoints, uint64_t* nleaves,
+                     uint64_t* nnodes, int* height) {
+  if (!rt) return SJCS_ERR_INVALID;
+  if (npoints) *npoints = rt->tree.points();
+  if (nleaves) *nleaves = rt->tree.leaves();
+  if (nnodes) *nnodes = rt->tree.nodes();
+  if (height) *height = rt->tree.height();
+  return SJCS_OK;
+}
+
+int sjcs_rtree_leaves(const sjcs_rtree* rt, uint64_t* ids, uint32_t* leaf_len) {
+  if (!rt || ((!ids || !leaf_len) && rt->tree.points())) return SJCS_ERR_INVALID;
+  if (rt->tree.points() == 0) return SJCS_OK;
+  rt->tree.leaf_ids(ids, leaf_len);
+  return SJCS_OK;
+}
+
+const char* sjcs_rtree_error(const sjcs_rtree* rt) { return rt ? rt->err.c_str() : ""; }
+
+int sjcs_rtree_count(sjcs_rtree* rt, const float* centrals, size_t stride_bytes, uint64_t n,
+                     const float* radiuses_sq, int nrads, uint64_t* counts) {
+  if (!rt) return SJCS_ERR_INVALID;
+  sjcs_engine* e = rt->e;
+  if (!e) return rt_fail(rt, SJCS_ERR_STATE, "a host-only R-tree cannot count (no engine)");
+  if ((!centrals && n) || stride_bytes < 12 || (!counts && n) || !radiuses_sq)
+    return rt_fail(rt, SJCS_ERR_INVALID, "null pointer or stride < 12");
+  if (nrads < 1 || nrads > kRtMaxShells)
+    return rt_fail(rt, SJCS_ERR_INVALID, "R-tree count: 1 to 1024 shells");
+  if (9 * rt->tree.height() + 1 > kRtStack)
+    return rt_fail(rt, SJCS_ERR_INVALID, "R-tree too deep for the count's stack");
+  int rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // the pinned staging is shared with the engine
+  if (rc == SJCS_OK) rc = rt_upload(rt);
+  if (rc != SJCS_OK) return rc;
+  if (n == 0) return SJCS_OK;
+  bool sorted = true;
+  for (int k = 1; k < nrads; ++k) sorted &= radiuses_sq[k - 1] <= radiuses_sq[k];
+  // MultiRangeCount.c:17: sqrt of the last threshold, in double, to FP32
+  const float rbox = (float)sqrt((double)radiuses_sq[nrads - 1]);
+  DevBuf pts, r2, dcounts;
+  rc = upload_points(e, pts, centrals, stride_bytes, n);
+  const size_t nc = (size_t)n * (size_t)nrads;
+  std::vector<uint64_t> host;
+  if (rc == SJCS_OK) {
+    hipError_t h = ensure(r2, sizeof(float) * nrads);
+    if (h == hipSuccess)
+      h = hipMemcpyAsync(r2.p, radiuses_sq, sizeof(float) * nrads, hipMemcpyHostToDevice,
+                         e->stream);
+    if (h == hipSuccess) h = ensure(dcounts, nc * sizeof(uint64_t));
+    if (h == hipSuccess) {
+      RtView v;
+      v.meta = rt->meta.as<uint32_t>();
+      v.lox = rt->lo[0].as<float>();
+      v.loy = rt->lo[1].as<float>();
+      v.loz = rt->lo[2].as<float>();
+      v.hix = rt->hi[0].as<float>();
+      v.hiy = rt->hi[1].as<float>();
+      v.hiz = rt->hi[2].as<float>();
+      v.child = rt->child.as<uint32_t>();
+      // the root is node 0 of the breadth-first layout
+      h = rt_count(v, 0, pts.as<float>(), n, r2.as<float>(), nrads, sorted, rbox,
+                   dcounts.as<uint64_t>(), e->stream);
+    }
+    if (h == hipSuccess) {
+      try {
+        host.resize(nc);
+      } catch (const std::bad_alloc&) {
+        h = hipErrorOutOfMemory;
+      }
+    }
+    if (h == hipSuccess)
+      h = hipMemcpyAsync(host.data(), dcounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         e->stream);
+    if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+    if (h != hipSuccess) rc = rt_hip_fail(rt, h, "R-tree count");
+  }
+  if (rc == SJCS_OK)
+    for (size_t i = 0; i < nc; ++i) counts[i] += host[i];
+  pts.release();
+  r2.release();
+  dcounts.release();
+  return rc;
+}
+
+void sjcs_rtree_destroy(sjcs_rtree* rt) {
+  if (!rt) return;
+  if (rt->e) {
+    (void)hipSetDevice(rt->e->device);
+    (void)hipStreamSynchronize(rt->e->stream);
+  }
+  rt->release();
+  delete rt;
 }
 
 }  // extern "C"

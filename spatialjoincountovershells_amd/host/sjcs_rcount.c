@@ -25,6 +25,15 @@
  * so "index destruction" is ~0.  With --compressed, each file's uint16
  * triplets go to the device as they are and are decoded there ("particles
  * decoding + index construction" = decode + cell sort).
+ *
+ * Run under a name that starts with "rt_rcount" (a link named rt_rcount),
+ * it is the drop-in for the reference's rt_rcount instead (src/rt_rcount/
+ * main.c, the same argv, stdout lines and result format): each particle
+ * file is inserted point by point into rt_rcount's R-tree on the host
+ * (include/sjcs.h, sjcs_rtree_*), and the halos are counted against it on
+ * the GPU with rt_rcount's leaf walk, whose counts differ from astr_rcount's
+ * for halos that sit exactly on a particle.  Files are handled one after the
+ * other with either --backlog setting.
  */
 #include <pthread.h>
 #include <stdio.h>
@@ -96,7 +105,7 @@ static void *setup_run(void *arg)
         s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[0]);
     if (s->rc == SJCS_OK)
         s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[1]);
-    if (s->rc == SJCS_OK)
+    if (s->rc == SJCS_OK && s->first)
         s->read_rc = read_chunk(s->e, s->pin, s->first, s->buf, s->compressed, s->nthreads,
                                 s->err, sizeof(s->err));
     return NULL;
@@ -274,6 +283,122 @@ static void finish_chunk(sjcs_engine *e, uint64_t n, int compressed)
     print_ms(">>>> index destruction time", &a);
 }
 
+/* ---- rt_rcount mode ---- */
+
+static int rt_named(const char *argv0)
+{
+    const char *b = strrchr(argv0, '/');
+    b = b ? b + 1 : argv0;
+    return strncmp(b, "rt_rcount", 9) == 0;
+}
+
+static void *grow_host(void *p, uint64_t *cap, uint64_t bytes)
+{
+    void *q;
+    if (bytes <= *cap)
+        return p;
+    q = realloc(p, (size_t)(bytes ? bytes : 16));
+    if (!q) {
+        fprintf(stderr, "ERROR: memory shortage\n");
+        exit(EXIT_FAILURE);
+    }
+    *cap = bytes;
+    return q;
+}
+
+/* The chunk loop of rt_rcount (src/rt_rcount/main.c:200-226, 618-673):
+ * per file, its points on the host (a compressed file is decoded on the
+ * device and copied back), an R-tree built by inserting them in order, the
+ * halos counted against it on the GPU (counts += ...), the tree dropped. */
+static void rt_chunks(sjcs_engine *e, void *pin[2], const sjcs_cli *cli, char **argv,
+                      const sjcs_halos *halos, const float *r2, uint64_t *counts, int nthreads)
+{
+    const int nfiles = cli->particle_end - cli->particle_beg;
+    const char *fn = cli->compressed ? "readCompressedParticles" : "readDecodedParticles";
+    chunk_buf cb;
+    float *xyz = NULL;
+    uint64_t cap = 0, dcap = 0;
+    void *d_xyz = NULL;
+    int i;
+    memset(&cb, 0, sizeof(cb));
+    for (i = 0; i < nfiles; i++) {
+        const char *fname = argv[cli->particle_beg + i];
+        struct timeval a, b;
+        sjcs_rtree *rt = NULL;
+        uint64_t n = 0;
+        long read_ms;
+        int rc;
+        TIMER_START(a);
+        if (cli->compressed) {
+            read_chunk_or_die(e, pin, fname, &cb, 1, nthreads);
+            n = cb.n;
+            read_ms = cb.read_ms;
+        } else {
+            if (sjcs_particle_count(fname, &n) != 0) {
+                fprintf(stderr, "ERROR: file %s not found in %s()\n", fname, fn);
+                exit(EXIT_FAILURE);
+            }
+            xyz = grow_host(xyz, &cap, 12 * n);
+            rc = n ? sjcs_pread_par(fname, 0, 12 * n, xyz, nthreads) : 0;
+            if (rc == -1) {
+                fprintf(stderr, "ERROR: file %s not found in %s()\n", fname, fn);
+                exit(EXIT_FAILURE);
+            }
+            if (rc != 0) {
+                fprintf(stderr, "ERROR: while reading %s in %s()\n", fname, fn);
+                exit(EXIT_FAILURE);
+            }
+            gettimeofday(&b, NULL);
+            read_ms = sjcs_elapsed_ms(&a, &b);
+        }
+        printf(">>>> particles reading time: %ld milli seconds\n", read_ms);
+        TIMER_START(a);
+        if (cli->compressed && n) {
+            if (dcap < 12 * n) {
+                if (d_xyz)
+                    sjcs_device_free(e, d_xyz);
+                d_xyz = NULL;
+                dcap = 0;
+                if (sjcs_device_alloc(e, (size_t)(12 * n), &d_xyz) != SJCS_OK)
+                    die_engine(e, "device memory for the decode");
+                dcap = 12 * n;
+            }
+            xyz = grow_host(xyz, &cap, 12 * n);
+            if (sjcs_decode_compressed_device(e, cb.d, n, &cb.dec, d_xyz) != SJCS_OK ||
+                sjcs_copy_to_host(e, xyz, d_xyz, (size_t)(12 * n)) != SJCS_OK)
+                die_engine(e, "particle decode");
+        }
+        if (sjcs_rtree_create(e, &rt) != SJCS_OK)
+            die_engine(e, "R-tree");
+        if (sjcs_rtree_insert(rt, xyz, n) != SJCS_OK) {
+            fprintf(stderr, "ERROR: R-tree: %s\n", sjcs_rtree_error(rt));
+            exit(EXIT_FAILURE);
+        }
+        print_ms(cli->compressed ? ">>>> particles decoding + index construction time"
+                                 : ">>>> index construction time", &a);
+        printf("  # of particles = %llu\n", (unsigned long long)n);
+        if (cli->maxbacklogs > 0)
+            printf(">>>> index getting time: %ld milli seconds\n", 0L);
+        TIMER_START(a);
+        if (sjcs_rtree_count(rt, halos->n ? halos->halo[0].point : NULL, sizeof(sjcs_halo),
+                             halos->n, r2, cli->nrads, counts) != SJCS_OK) {
+            fprintf(stderr, "ERROR: search: %s\n", sjcs_rtree_error(rt));
+            exit(EXIT_FAILURE);
+        }
+        print_ms(">>>> searching time", &a);
+        TIMER_START(a);
+        sjcs_rtree_destroy(rt);
+        print_ms(">>>> index destruction time", &a);
+        trace("file counted (R-tree)");
+        fflush(stdout);
+    }
+    if (cb.d)
+        sjcs_device_free(e, cb.d);
+    if (d_xyz)
+        sjcs_device_free(e, d_xyz);
+    free(xyz);
+}
+
 /* ---- counts -> result writer ---- */
 
 typedef struct {
@@ -306,12 +431,13 @@ int main(int argc, char *argv[])
     setup_job setup;
     pthread_t setup_th;
     row_source src;
-    int nfiles, i, hthreads, rc, setup_threaded;
+    int nfiles, i, hthreads, rc, setup_threaded, rt_mode;
     const char *dev_env = getenv("SJCS_DEVICE");
     const char *trace_env = getenv("SJCS_TRACE");
 
     gettimeofday(&g_t0, NULL);
     g_trace = trace_env != NULL && trace_env[0] == '1';
+    rt_mode = rt_named(argv[0]);
     sjcs_cli_parse(argc, argv, &cli);
     sjcs_cli_print_info(stdout, &cli, argv);
     if (cli.errflag) {
@@ -336,7 +462,8 @@ int main(int argc, char *argv[])
     memset(&setup, 0, sizeof(setup));
     hthreads = cli.nthreads < 256 ? cli.nthreads : 256;
     setup.device = dev_env ? atoi(dev_env) : 0;
-    setup.first = argv[cli.particle_beg];
+    /* rt mode reads its files into host memory, one at a time */
+    setup.first = rt_mode ? NULL : argv[cli.particle_beg];
     setup.buf = &bufs[0];
     setup.compressed = cli.compressed;
     /* half the threads: the halo parse runs meanwhile */
@@ -378,6 +505,33 @@ int main(int argc, char *argv[])
     if (setup.rc != SJCS_OK)
         die_engine(e, "GPU engine");
     trace("device set up");
+    if (rt_mode) {
+        uint64_t *counts = calloc(halos.n ? (size_t)halos.n * (size_t)cli.nrads : 1,
+                                  sizeof(uint64_t));
+        if (!counts) {
+            fprintf(stderr, "ERROR: memory shortage\n");
+            exit(EXIT_FAILURE);
+        }
+        TIMER_START(tvtotal);
+        puts(cli.maxbacklogs == 0 ? "  -- with no backlog execution"
+                                  : "  -- with backlog execution");
+        fflush(stdout);
+        rt_chunks(e, setup.pin, &cli, argv, &halos, r2, counts, hthreads);
+        print_ms(">>>> total searching time", &tvtotal);
+        TIMER_START(tv);
+        if (sjcs_write_result(cli.outfile, &halos, counts, r2, cli.nrads, hthreads) < 0)
+            exit(EXIT_FAILURE);
+        print_ms(">>>> result writing time", &tv);
+        free(counts);
+        free(bufs);
+        sjcs_host_free(setup.pin[0]);
+        sjcs_host_free(setup.pin[1]);
+        sjcs_engine_destroy(e);
+        sjcs_free_halos(&halos);
+        free(rad);
+        free(r2);
+        return 0;
+    }
     if (sjcs_set_shells(e, r2, cli.nrads) != SJCS_OK)
         die_engine(e, "shells");
     if (sjcs_set_halos(e, halos.n ? halos.halo[0].point : NULL, sizeof(sjcs_halo), halos.n) !=

@@ -155,6 +155,7 @@ def main():
             if r2.returncode == 0:
                 write_gz(out_rt, os.path.join(GOLD, name + ".rt_expected.gz"))
                 entry["rt_expected"] = name + ".rt_expected.gz"
+                entry["rt_stdout_masked"] = mask_stdout(r2.stdout).replace(GOLD + "/", "")
         cases.append(entry)
         print(f"{name}: ok", flush=True)
 
@@ -178,9 +179,9 @@ def main():
         h[i, ax[i]] = rng.uniform(0, 5) if lo[i] else rng.uniform(995, 1000)
     h[:4] = [[0, 0, 0], [1000, 1000, 1000], [0.0005, 999.9995, 500], [2.5, 2.5, 997.5]]
     write_halos(os.path.join(GOLD, "uni_halos.list"), h)
-    case("uni_8", ["uni_particles.dat"], "uni_halos.list", "0.001:5:8")
+    case("uni_8", ["uni_particles.dat"], "uni_halos.list", "0.001:5:8", rt=True)
     case("uni_40", ["uni_particles.dat"], "uni_halos.list", "0.001:5:40")
-    case("uni_dense_16", ["uni_particles.dat"], "uni_halos.list", "0.5:100:16")
+    case("uni_dense_16", ["uni_particles.dat"], "uni_halos.list", "0.5:100:16", rt=True)
     case("uni_k1", ["uni_particles.dat"], "uni_halos.list", "3:7:1")
     case("uni_big_600", ["uni_particles.dat"], "uni_halos.list", "1:600:5")
 
@@ -188,7 +189,8 @@ def main():
     for i in range(3):
         write_particles(os.path.join(GOLD, f"uni_part{i}.dat"),
                         p[i * 6000:(i + 1) * 6000])
-    case("uni_3files", [f"uni_part{i}.dat" for i in range(3)], "uni_halos.list", "0.001:5:8")
+    case("uni_3files", [f"uni_part{i}.dat" for i in range(3)], "uni_halos.list", "0.001:5:8",
+         rt=True)
 
     # 4. ties and faces: dyadic grid points, so distances and box edges are
     #    exact; halos on grid points see particles exactly on their query
@@ -205,9 +207,9 @@ def main():
                    [999.0, 1.0, 2.0], [500.0, 0.0, 1000.0]])
     write_halos(os.path.join(GOLD, "tie_halos.list"), th)
     # radius 2^-2 .. 2^1: r2 = {1/16, .., 4} dyadic, r_box = 2 exactly
-    case("tie_dyadic", ["tie_particles.dat"], "tie_halos.list", "0.25:2:4")
+    case("tie_dyadic", ["tie_particles.dat"], "tie_halos.list", "0.25:2:4", rt=True)
     case("tie_dyadic_7", ["tie_particles.dat"], "tie_halos.list", "0.25:2:7")
-    case("tie_mixed", ["tie_particles.dat"], "tie_halos.list", "0.1:1.75:9")
+    case("tie_mixed", ["tie_particles.dat"], "tie_halos.list", "0.1:1.75:9", rt=True)
 
     # 5. the reference's unreferenced irregular point set (negative and
     #    out-of-box coordinates), converted to the binary format
@@ -222,7 +224,7 @@ def main():
         write_particles(os.path.join(GOLD, "rtree_particles.dat"), rp)
         rh = np.vstack([rp[::97] + np.float32(0.01), [[0, 0, 0], [1, 1, 1], [999.5, 999.5, 0.5]]])
         write_halos(os.path.join(GOLD, "rtree_halos.list"), rh)
-        case("rtree_4", ["rtree_particles.dat"], "rtree_halos.list", "0.05:4:12")
+        case("rtree_4", ["rtree_particles.dat"], "rtree_halos.list", "0.05:4:12", rt=True)
 
     # 6. halo text edge cases: special values, odd number formats, and a last
     #    line without '\n' (the reference drops it)
@@ -236,7 +238,7 @@ def main():
            "905 0 1e+12 0 0 0 0 0 260 260 260 0"]
     write_halos(os.path.join(GOLD, "fmt_halos.list"), h[:50], raw_lines=raw,
                 trailing_newline=False)
-    case("fmt_halos", ["uni_particles.dat"], "fmt_halos.list", "0.5:30:10")
+    case("fmt_halos", ["uni_particles.dat"], "fmt_halos.list", "0.5:30:10", rt=True)
 
     # 7. --filter-file: unsorted ids with repeats and ids not present
     ids = list(range(0, 400, 3)) + [7, 7, 5000, 1]
@@ -257,7 +259,7 @@ def main():
     # a whole 24^3 grid, spacing 1000/24 (inexact in FP32), displacements
     # +-30: particles below 0 and near 1000 on every axis
     write_compressed(os.path.join(GOLD, "gad_grid.gdt"), uv_random(24 ** 3), 24 ** 3)
-    case("gad_grid", ["gad_grid.gdt"], "uni_halos.list", "0.5:20:12", extra=cz)
+    case("gad_grid", ["gad_grid.gdt"], "uni_halos.list", "0.5:20:12", extra=cz, rt=True)
     # one 32^3 snapshot in two files with their own displacement encodings
     uv2 = uv_random(32 ** 3, 12)
     write_compressed(os.path.join(GOLD, "gad_part0.gdt"), uv2[:16384], 32 ** 3,
@@ -265,7 +267,7 @@ def main():
     write_compressed(os.path.join(GOLD, "gad_part1.gdt"), uv2[16384:], 32 ** 3,
                      disp_min=-12.5, disp_max=7.75, pos_bits=12, id_start=16385)
     case("gad_2files", ["gad_part0.gdt", "gad_part1.gdt"], "uni_halos.list", "0.5:30:10",
-         extra=cz)
+         extra=cz, rt=True)
     # edges: id 0 (grid coordinate -1), ids past the grid (a second tile:
     # x beyond the box), extreme triplets; a half-size box with a negative
     # displacement span and 10-bit triplets
@@ -300,6 +302,32 @@ def main():
         raise RuntimeError(f"gad_fma: only {hf.shape[0]} sensitive halos")
     write_halos(os.path.join(GOLD, "gad_fma_halos.list"), hf, fmt="%.9g")
     case("gad_fma", ["gad_fma.gdt"], "gad_fma_halos.list", "0.5:4:8", extra=cz)
+
+    # 10. rt_rcount's leaf semantics: halos exactly on particles (a distance
+    #     of 0 ends the scan of that particle's leaf), duplicated particles,
+    #     clumps dense enough for deep trees, halos near faces.  Its own
+    #     generator, so the fixtures above stay as they were.
+    rrng = np.random.default_rng(20261205)
+    cen = rrng.random((300, 3)) * 1000.0
+    clump = cen[rrng.integers(0, 300, 15000)] + rrng.normal(0.0, 2.0, (15000, 3))
+    rp = np.vstack([np.mod(clump, 1000.0), rrng.random((15000, 3)) * 1000.0])
+    rp = rp.astype(np.float32)
+    dup = rp[rrng.integers(0, rp.shape[0], 600)]
+    rp = np.insert(rp, rrng.integers(0, rp.shape[0], 600), dup, axis=0)
+    write_particles(os.path.join(GOLD, "rt_particles.dat"), rp)
+    near = rp[rrng.integers(0, rp.shape[0], 200)].astype(np.float64)
+    face = rrng.integers(0, 3, 200)
+    for i in range(200):
+        near[i, face[i]] = rrng.uniform(0, 3) if i % 2 else rrng.uniform(997, 1000)
+    rh = np.vstack([rp[rrng.integers(0, rp.shape[0], 1500)].astype(np.float64),
+                    cen[:400], rrng.random((700, 3)) * 1000.0, near])
+    write_halos(os.path.join(GOLD, "rt_halos.list"), rh, fmt="%.9g")
+    case("rt_onpart", ["rt_particles.dat"], "rt_halos.list", "0.01:3:10", rt=True)
+    case("rt_onpart_wide", ["rt_particles.dat"], "rt_halos.list", "0.2:40:6", rt=True)
+    # the same particles cut into two files: each file is its own tree
+    write_particles(os.path.join(GOLD, "rt_part0.dat"), rp[:11000])
+    write_particles(os.path.join(GOLD, "rt_part1.dat"), rp[11000:])
+    case("rt_2files", ["rt_part0.dat", "rt_part1.dat"], "rt_halos.list", "0.01:3:10", rt=True)
 
     # 8. option errors: stderr and exit status (argv[0] shown as ARGV0)
     errors = []

@@ -72,6 +72,17 @@ def oracle():
     lib.oracle_gadget_decode.argtypes = [vp, ctypes.c_uint64, ll, ll, ctypes.c_int,
                                          ctypes.c_float, ctypes.c_float, ctypes.c_double,
                                          ctypes.c_int, vp]
+    lib.oracle_rt_build.restype = vp
+    lib.oracle_rt_build.argtypes = [vp, ctypes.c_uint64]
+    lib.oracle_rt_free.restype = None
+    lib.oracle_rt_free.argtypes = [vp]
+    lib.oracle_rt_height.restype = ctypes.c_int
+    lib.oracle_rt_height.argtypes = [vp]
+    lib.oracle_rt_leaves.restype = ctypes.c_uint64
+    lib.oracle_rt_leaves.argtypes = [vp, vp, vp]
+    lib.oracle_rt_count.restype = None
+    lib.oracle_rt_count.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_uint64, vp, ctypes.c_int,
+                                    vp]
     _oracle = lib
     return lib
 
@@ -92,6 +103,53 @@ def oracle_count(particles, halos, r2, threads=8):
     w = oracle().oracle_count(p.ctypes.data, p.shape[0], h.ctypes.data, 12, h.shape[0],
                               r2.ctypes.data, r2.size, c.ctypes.data, threads)
     return c, w
+
+
+class OracleRTree:
+    """rt_rcount's R-tree over `particles`, inserted in order (the oracle's
+    restatement of src/objects/RTree/Build.c)."""
+
+    def __init__(self, particles):
+        self.p = np.ascontiguousarray(particles, np.float32).reshape(-1, 3)
+        self.t = oracle().oracle_rt_build(self.p.ctypes.data, self.p.shape[0])
+        if not self.t:
+            raise ValueError("oracle R-tree build failed")
+
+    def close(self):
+        if self.t:
+            oracle().oracle_rt_free(self.t)
+            self.t = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def height(self):
+        return oracle().oracle_rt_height(self.t)
+
+    def leaves(self):
+        """(ids in leaf order, leaf lengths), leaves depth first."""
+        n = self.p.shape[0]
+        ids = np.zeros(max(n, 1), np.uint64)
+        lens = np.zeros(max(n, 1), np.uint32)
+        nl = oracle().oracle_rt_leaves(self.t, ids.ctypes.data, lens.ctypes.data)
+        return ids[:n], lens[:nl]
+
+    def count(self, halos, r2):
+        h = np.ascontiguousarray(halos, np.float32).reshape(-1, 3)
+        r2 = np.ascontiguousarray(r2, np.float32)
+        c = np.zeros((h.shape[0], r2.size), np.uint64)
+        oracle().oracle_rt_count(self.t, h.ctypes.data, 12, h.shape[0], r2.ctypes.data,
+                                 r2.size, c.ctypes.data)
+        return c
+
+
+def oracle_rt_count(particles, halos, r2):
+    """rt_rcount's counts [nh][K] for one particle file."""
+    with OracleRTree(particles) as t:
+        return t.count(halos, r2)
 
 
 def shell_index(dist, r2):

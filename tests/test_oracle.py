@@ -291,3 +291,96 @@ def test_gadget_fma_case_tells_the_decodes_apart():
     c0, _ = t.oracle_count(plain, pts, r2)
     assert pts.shape[0] >= 10
     assert (c0 != ecnt).any(axis=1).all()
+
+
+# ------------------------------------------------ rt_rcount's R-tree count --
+
+RT_CASES = [c for c in CASES if "rt_expected" in c]
+
+
+def oracle_rt_case_counts(case):
+    mn, mx, k = t.parse_radius(case["radius"])
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    chunks, ids, pts = t.case_inputs(case)
+    tot = np.zeros((pts.shape[0], k), np.uint64)
+    for ch in chunks:
+        tot += t.oracle_rt_count(ch, pts, r2)
+    return ids, tot, k
+
+
+@pytest.mark.parametrize("case", RT_CASES, ids=[c["name"] for c in RT_CASES])
+def test_rt_oracle_matches_reference_rt_rcount(case):
+    """The R-tree restatement against rt_rcount's own results (oracle/_ref,
+    built from the reference sources), cases where its counts differ from
+    astr_rcount's among them."""
+    ids, got, k = oracle_rt_case_counts(case)
+    eids, ecnt, _, _ = t.parse_result(t.read_expected_text(case["rt_expected"]), k)
+    assert np.array_equal(eids, ids)
+    assert np.array_equal(got, ecnt)
+
+
+def test_rt_fixtures_exercise_the_leaf_rule():
+    """The rt fixtures hold rows where the two CLIs disagree (a halo on a
+    particle), so they pin the leaf rule and not only the shared part."""
+    differing = 0
+    for case in RT_CASES:
+        _, astr, k = oracle_case_counts(case)
+        _, ecnt, _, _ = t.parse_result(t.read_expected_text(case["rt_expected"]), k)
+        differing += int((astr != ecnt).any(axis=1).sum())
+    assert differing >= 1500
+
+
+def test_rt_equals_astr_off_the_particles():
+    """With no halo on a particle (and no box-edge rounding case), the leaf
+    rule never fires and an R-tree walk that prunes by exact boxes finds the
+    pairs the brute force finds: periodic images included."""
+    rng = np.random.default_rng(11)
+    p = rng.random((30000, 3), dtype=np.float32) * np.float32(1000)
+    h = rng.random((3000, 3), dtype=np.float32) * np.float32(1000)
+    h[:300, 0] = rng.uniform(0, 8, 300).astype(np.float32)
+    h[300:600, 1] = rng.uniform(992, 1000, 300).astype(np.float32)
+    h[600:700, :] = rng.uniform(0, 8, (100, 3)).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.5, 30.0, 12)
+    want, _ = t.oracle_count(p, h, r2)
+    assert want.sum() > 5000
+    assert np.array_equal(t.oracle_rt_count(p, h, r2), want)
+
+
+def test_rt_zero_distance_ends_the_leaf_scan():
+    """Ten points fill the root leaf in insertion order; a halo on point 4
+    counts points 0-3 only (src/objects/RTree/MultiRangeCount.c:104-105),
+    where astr_rcount counts all ten, point 4 itself in the first shell."""
+    xs = np.float32(500.0) + np.arange(10, dtype=np.float32) * np.float32(0.125)
+    p = np.stack([xs, np.full(10, 500, np.float32), np.full(10, 500, np.float32)], 1)
+    r2 = np.array([0.0625, 0.25, 1.0, 4.0], np.float32)  # r = 1/4, 1/2, 1, 2
+    h = p[[4]]
+    # distances 0.5, 0.375, 0.25, 0.125 -> squared .25, .140625, .0625, .015625
+    want_rt = np.array([[1, 2, 1, 0]], np.uint64)
+    with t.OracleRTree(p) as tr:
+        assert tr.height() == 0
+        got = tr.count(h, r2)
+    assert np.array_equal(got, want_rt)
+    astr, _ = t.oracle_count(p, h, r2)
+    # astr: 0 (itself) and 0.125 -> shell 0; 0.25 (x2) -> shell 1 (not < .0625);
+    # 0.375, 0.5 (x2 each side where present) -> shells 2, 2; 0.625 -> 2
+    d2 = ((p[:, 0] - h[0, 0]) ** 2).astype(np.float32)
+    want_astr = np.zeros((1, 4), np.uint64)
+    for v in d2:
+        want_astr[0, int(np.argmax(v < r2))] += 1 if (v < r2).any() else 0
+    assert np.array_equal(astr, want_astr)
+    assert astr.sum() == 10 and got.sum() == 4
+
+
+def test_rt_tree_shape():
+    """Structure the build must keep: every id once, leaf sizes within
+    [MIN_ENT, MAX_ENT] below the root, the height of a balanced tree."""
+    rng = np.random.default_rng(3)
+    p = rng.random((50000, 3), dtype=np.float32) * np.float32(1000)
+    with t.OracleRTree(p) as tr:
+        ids, lens = tr.leaves()
+        h = tr.height()
+    assert np.array_equal(np.sort(ids), np.arange(p.shape[0], dtype=np.uint64))
+    assert lens.sum() == p.shape[0]
+    assert lens.min() >= 4 and lens.max() <= 10
+    # at least 4 and at most 10 children per inner node
+    assert np.ceil(np.log(lens.size) / np.log(10)) <= h <= np.floor(np.log(lens.size) / np.log(4)) + 1
