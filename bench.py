@@ -71,6 +71,10 @@ CONFIGS = {
                          "displacements over +-20) x 1e7 uniform halos, --radius 0.001:5:40, "
                          "decoded on the device",
                 np=464 ** 3, nh=10_000_000, radius=(0.001, 5.0, 40), chunks=1, compressed=464),
+    "rt": dict(workload="rt_rcount mode: uniform 1e7 particles in one R-tree (built on the host "
+                        "in insertion order) x 1e6 uniform halos (a tenth exactly on particles), "
+                        "--radius 0.001:5:40",
+               np=10_000_000, nh=1_000_000, radius=(0.001, 5.0, 40), chunks=1, rtree=True),
 }
 
 
@@ -81,7 +85,9 @@ METRICS = {
     "c4": "shell pair-tests/s (5M clustered halos x 200M clustered particles x 64 shells, 1 GPU)",
     "c5": "served requests/s (3e8-particle resident index, batches of 1-1e5 halos, own shells)",
     "c2z": "shell pair-tests/s (10M halos x 99.9M compressed particles x 40 shells, 1 GPU)",
+    "rt": "rt_rcount halo queries/s (1M halos x 10M-particle R-tree x 40 shells, 1 GPU)",
 }
+RT_REF_BIN = os.path.join(ROOT, "oracle", "_ref", "rt_rcount")
 
 
 def log(msg):
@@ -363,6 +369,119 @@ def serve_bench(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def rt_bench(args, cfg):
+    """rt_rcount mode: one R-tree over the particles, built on the host point
+    by point as rt_rcount builds it, then every halo counted against it on
+    the GPU (sjcs_rtree_count_device: centres and u64 counts resident in
+    HBM).  A step is one count of all halos; the tree's upload is the first
+    warm-up step.  The CPU baseline is the reference rt_rcount (built from
+    its sources) on a halo sample of the same set, whose result rows must
+    equal the GPU's."""
+    rng = np.random.default_rng(args.seed)
+    mn, mx, k = cfg["radius"]
+    _, r2 = sj.shell_thresholds(mn, mx, k)
+    log(f"[bench] generating {cfg['workload']}")
+    parts = synth.uniform_points(rng, cfg["np"])
+    halos = synth.uniform_points(rng, cfg["nh"])
+    on = rng.choice(cfg["nh"], cfg["nh"] // 10, replace=False)
+    halos[on] = parts[rng.integers(0, cfg["np"], on.size)]
+    nh = halos.shape[0]
+    eng = sj.Engine(0)
+    rt = sj.RTree(eng)
+    t0 = time.perf_counter()
+    rt.insert(parts)
+    build_s = time.perf_counter() - t0
+    shape = rt.shape()
+    log(f"[bench] host R-tree build {build_s:.2f} s: {shape}")
+    d_h = eng.device_array(halos)
+    d_c = eng.device_array(np.zeros((nh, k), np.uint64))
+    t0 = time.perf_counter()
+    rt.multi_range_count_device(d_h, nh, r2, d_c)
+    first_s = time.perf_counter() - t0
+    for _ in range(max(args.warmup - 1, 0)):
+        rt.multi_range_count_device(d_h, nh, r2, d_c)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rt.multi_range_count_device(d_h, nh, r2, d_c)  # returns when the counts are added
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * sum(times) / len(times)
+    eng.device_free(d_c)
+    counts = rt.multi_range_count(halos, r2)
+    eng.device_free(d_h)
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(RT_REF_BIN):
+        log("[bench] cpu baseline (reference rt_rcount on a halo sample)")
+        cpu = rt_cpu_baseline(parts, halos, counts, cfg["radius"], args)
+    rt.close()
+    eng.close()
+    out = {
+        "metric": METRICS[args.config],
+        "value": nh / (ms / 1e3),
+        "unit": "halos/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "particles": cfg["np"], "halos": nh,
+                   "shells": k, "radius": f"{mn}:{mx}:{k}", "pairs_counted": int(counts.sum()),
+                   "parallelism": "single GPU"},
+        "tree": dict(shape, host_build_s=round(build_s, 3),
+                     first_count_with_upload_s=round(first_s, 3)),
+        "roofline": None,
+        "roofline_note": "a walk of dependent node loads: latency-bound, no byte or flop "
+                         "roofline applies",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def rt_cpu_baseline(parts, halos, counts, radius, args):
+    tmp = tempfile.mkdtemp(prefix="sjcs_rt_")
+    try:
+        pf = os.path.join(tmp, "part0.dat")
+        parts.tofile(pf)
+        nsample = min(args.cpu_halos, 20_000, halos.shape[0])
+        pick = np.sort(np.random.default_rng(7).choice(halos.shape[0], nsample, replace=False))
+        hf = os.path.join(tmp, "halo.list")
+        write_halo_file(hf, halos[pick])
+        out = os.path.join(tmp, "out.txt")
+        mn, mx, k = radius
+        threads = args.cpu_threads
+        cmd = [RT_REF_BIN, "--thread", str(threads), "--next-setup", str(threads),
+               "--backlog", "0", "--particle-files", pf, "--halo-file", hf,
+               "--radius", f"{mn}:{mx}:{k}", "--output-file", out]
+        t0 = time.time()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        wall = time.time() - t0
+        m = re.findall(r">>>> searching time: (\d+) milli seconds", r.stdout)
+        b = re.findall(r">>>> index construction time: (\d+) milli seconds", r.stdout)
+        if r.returncode != 0 or not m:
+            log(f"rt cpu baseline failed rc={r.returncode}: {r.stderr[-300:]}")
+            return None
+        with open(out, "rb") as f:
+            text = f.read().decode()
+        rows = [int(v) for v in re.findall(r": (\d+)\n", text.split("\nTotal\n")[0])]
+        same = np.array_equal(np.array(rows, np.uint64).reshape(nsample, k), counts[pick])
+        if not same:
+            raise SystemExit("bench.py: rt counts differ from the reference rt_rcount's")
+        search_s = max(int(m[0]), 1) / 1000.0
+        return {"value": nsample / search_s, "unit": "halos/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"rt_rcount --thread {threads}: all {parts.shape[0]} particles x "
+                          f"{nsample} halos (random sample of the same set), {mn}:{mx}:{k}; "
+                          f"{search_s:.3f} s of 'searching time', its single-threaded index "
+                          f"construction {int(b[0]) / 1000.0 if b else float('nan'):.1f} s "
+                          f"(process wall {wall:.1f} s); its rows equal the GPU's"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +537,8 @@ def main():
     cfg = CONFIGS[args.config]
     if cfg.get("serving"):
         return serve_bench(args, cfg)
+    if cfg.get("rtree"):
+        return rt_bench(args, cfg)
     mn, mx, k = cfg["radius"]
     _, r2 = sj.shell_thresholds(mn, mx, k)
     log(f"[bench] generating {cfg['workload']}")

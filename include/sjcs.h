@@ -382,6 +382,11 @@ int sjcs_rtree_leaves(const sjcs_rtree *rt, uint64_t *ids, uint32_t *leaf_len);
  * GPU (the tree is copied there on the first count after an insert). */
 int sjcs_rtree_count(sjcs_rtree *rt, const float *centrals, size_t stride_bytes, uint64_t n,
                      const float *radiuses_sq, int nrads, uint64_t *counts);
+/* The same with the centres (n packed FP32 xyz) and the u64 counts
+ * [n][nrads] on the device (d_counts += ...); radiuses_sq on the host.
+ * Returns when the counts are added. */
+int sjcs_rtree_count_device(sjcs_rtree *rt, const float *d_centrals, uint64_t n,
+                            const float *radiuses_sq, int nrads, uint64_t *d_counts);
 /* Message of the tree's last error (also sjcs_last_error of its engine). */
 const char *sjcs_rtree_error(const sjcs_rtree *rt);
 void sjcs_rtree_destroy(sjcs_rtree *rt);

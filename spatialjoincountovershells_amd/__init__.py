@@ -56,7 +56,7 @@ EXPORTED = (
     "sjcs_server_create", "sjcs_server_submit", "sjcs_server_test", "sjcs_server_wait",
     "sjcs_server_times", "sjcs_server_release", "sjcs_server_error", "sjcs_server_destroy",
     "sjcs_rtree_create", "sjcs_rtree_insert", "sjcs_rtree_shape", "sjcs_rtree_leaves",
-    "sjcs_rtree_count", "sjcs_rtree_error", "sjcs_rtree_destroy",
+    "sjcs_rtree_count", "sjcs_rtree_count_device", "sjcs_rtree_error", "sjcs_rtree_destroy",
 )
 
 
@@ -211,6 +211,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                    ctypes.POINTER(u64), ctypes.POINTER(i32)]),
         "sjcs_rtree_leaves": (i32, [vp, vp, vp]),
         "sjcs_rtree_count": (i32, [vp, vp, sz, u64, vp, i32, vp]),
+        "sjcs_rtree_count_device": (i32, [vp, vp, u64, vp, i32, vp]),
         "sjcs_rtree_error": (ctypes.c_char_p, [vp]),
         "sjcs_rtree_destroy": (None, [vp]),
     }
@@ -487,6 +488,14 @@ class RTree:
         self._check(self._lib.sjcs_rtree_count(self._h, _ptr(c), 12, c.shape[0], _ptr(r2),
                                                r2.size, _ptr(counts)))
         return counts
+
+    def multi_range_count_device(self, d_centrals: int, n: int, radiuses_sq, d_counts: int):
+        """sjcs_rtree_count_device: centres (packed xyz) and u64 counts
+        [n][K] already on the device (counts +=)."""
+        r2 = _shells(radiuses_sq)
+        self._check(self._lib.sjcs_rtree_count_device(self._h, ctypes.c_void_p(d_centrals), n,
+                                                      _ptr(r2), r2.size,
+                                                      ctypes.c_void_p(d_counts)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -2458,9 +2458,9 @@ struct sjcs_rtree {
   RtTree tree;
   std::string err;
   bool on_device = false;  // the device arrays hold the current tree
-  DevBuf meta, lo[3], hi[3], child;
+  DevBuf lo[3], hi[3], child;
+  uint32_t root = 0, inner = 0;  // the root's reference, the inner nodes
   void release() {
-    meta.release();
     child.release();
     for (int d = 0; d < 3; ++d) {
       lo[d].release();
@@ -2501,10 +2501,10 @@ int rt_upload(sjcs_rtree* rt) {
   } catch (const std::bad_alloc&) {
     return rt_fail(rt, SJCS_ERR_NOMEM, "host memory for the R-tree layout");
   }
-  if (f.inner != 0 && (f.meta[0] & kRtLeaf))
-    return rt_fail(rt, SJCS_ERR_STATE, "R-tree layout: a leaf above an inner node");
+  if (rt->tree.nodes() > kRtNodeMask)
+    return rt_fail(rt, SJCS_ERR_INVALID, "R-tree with more than 2^28-1 nodes");
   rt->release();
-  hipError_t h = upload_vec(rt->meta, f.meta, e->stream);
+  hipError_t h = hipSuccess;
   for (int d = 0; d < 3 && h == hipSuccess; ++d) {
     h = upload_vec(rt->lo[d], f.lo[d], e->stream);
     if (h == hipSuccess) h = upload_vec(rt->hi[d], f.hi[d], e->stream);
@@ -2516,6 +2516,8 @@ int rt_upload(sjcs_rtree* rt) {
     rt->release();
     return rt_hip_fail(rt, h, "R-tree upload");
   }
+  rt->root = f.root;
+  rt->inner = f.inner;
   rt->on_device = true;
   return SJCS_OK;
 }
@@ -2570,39 +2572,40 @@ int sjcs_rtree_leaves(const sjcs_rtree* rt, uint64_t* ids, uint32_t* leaf_len) {
 
 const char* sjcs_rtree_error(const sjcs_rtree* rt) { return rt ? rt->err.c_str() : ""; }
 
-int sjcs_rtree_count(sjcs_rtree* rt, const float* centrals, size_t stride_bytes, uint64_t n,
-                     const float* radiuses_sq, int nrads, uint64_t* counts) {
-  if (!rt) return SJCS_ERR_INVALID;
-  sjcs_engine* e = rt->e;
-  if (!e) return rt_fail(rt, SJCS_ERR_STATE, "a host-only R-tree cannot count (no engine)");
-  if ((!centrals && n) || stride_bytes < 12 || (!counts && n) || !radiuses_sq)
-    return rt_fail(rt, SJCS_ERR_INVALID, "null pointer or stride < 12");
+}  // extern "C"
+
+namespace {
+
+int rt_check(sjcs_rtree* rt, int nrads) {
+  if (!rt->e) return rt_fail(rt, SJCS_ERR_STATE, "a host-only R-tree cannot count (no engine)");
   if (nrads < 1 || nrads > kRtMaxShells)
     return rt_fail(rt, SJCS_ERR_INVALID, "R-tree count: 1 to 1024 shells");
   if (9 * rt->tree.height() + 1 > kRtStack)
     return rt_fail(rt, SJCS_ERR_INVALID, "R-tree too deep for the count's stack");
-  int rc = use_device(e);
-  if (rc == SJCS_OK) rc = quiesce(e);  // the pinned staging is shared with the engine
-  if (rc == SJCS_OK) rc = rt_upload(rt);
-  if (rc != SJCS_OK) return rc;
-  if (n == 0) return SJCS_OK;
+  return SJCS_OK;
+}
+
+// d_out[h * nrads + k] += the shell-k count of centre h, for n centres
+// (packed xyz) at d_pts, both on the device; returns when done.
+int rt_count_into(sjcs_rtree* rt, const float* d_pts, uint64_t n, const float* radiuses_sq,
+                  int nrads, uint64_t* d_out) {
+  sjcs_engine* e = rt->e;
   bool sorted = true;
   for (int k = 1; k < nrads; ++k) sorted &= radiuses_sq[k - 1] <= radiuses_sq[k];
   // MultiRangeCount.c:17: sqrt of the last threshold, in double, to FP32
   const float rbox = (float)sqrt((double)radiuses_sq[nrads - 1]);
-  DevBuf pts, r2, dcounts;
-  rc = upload_points(e, pts, centrals, stride_bytes, n);
-  const size_t nc = (size_t)n * (size_t)nrads;
-  std::vector<uint64_t> host;
+  DevBuf r2, dcounts;
+  HaloState hs;
+  // the queries in Morton order: neighbours visit the same nodes
+  int rc = sort_halos(e, hs, d_pts, 12, n, false, false);
   if (rc == SJCS_OK) {
     hipError_t h = ensure(r2, sizeof(float) * nrads);
     if (h == hipSuccess)
       h = hipMemcpyAsync(r2.p, radiuses_sq, sizeof(float) * nrads, hipMemcpyHostToDevice,
                          e->stream);
-    if (h == hipSuccess) h = ensure(dcounts, nc * sizeof(uint64_t));
+    if (h == hipSuccess) h = ensure(dcounts, (size_t)n * (size_t)nrads * sizeof(uint32_t));
     if (h == hipSuccess) {
       RtView v;
-      v.meta = rt->meta.as<uint32_t>();
       v.lox = rt->lo[0].as<float>();
       v.loy = rt->lo[1].as<float>();
       v.loz = rt->lo[2].as<float>();
@@ -2610,28 +2613,81 @@ int sjcs_rtree_count(sjcs_rtree* rt, const float* centrals, size_t stride_bytes,
       v.hiy = rt->hi[1].as<float>();
       v.hiz = rt->hi[2].as<float>();
       v.child = rt->child.as<uint32_t>();
-      // the root is node 0 of the breadth-first layout
-      h = rt_count(v, 0, pts.as<float>(), n, r2.as<float>(), nrads, sorted, rbox,
-                   dcounts.as<uint64_t>(), e->stream);
-    }
-    if (h == hipSuccess) {
-      try {
-        host.resize(nc);
-      } catch (const std::bad_alloc&) {
-        h = hipErrorOutOfMemory;
-      }
+      h = rt_count(v, rt->root, rt->inner, hs.bx.as<float>(), hs.by.as<float>(),
+                   hs.bz.as<float>(), n, r2.as<float>(), nrads, sorted, rbox,
+                   dcounts.as<uint32_t>(), e->stream);
     }
     if (h == hipSuccess)
-      h = hipMemcpyAsync(host.data(), dcounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                         e->stream);
+      h = rt_add_counts(dcounts.as<uint32_t>(), hs.inv.as<uint32_t>(), d_out, n, nrads,
+                        e->stream);
     if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
     if (h != hipSuccess) rc = rt_hip_fail(rt, h, "R-tree count");
+  }
+  r2.release();
+  dcounts.release();
+  hs.release();
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sjcs_rtree_count(sjcs_rtree* rt, const float* centrals, size_t stride_bytes, uint64_t n,
+                     const float* radiuses_sq, int nrads, uint64_t* counts) {
+  if (!rt) return SJCS_ERR_INVALID;
+  if ((!centrals && n) || stride_bytes < 12 || (!counts && n) || !radiuses_sq)
+    return rt_fail(rt, SJCS_ERR_INVALID, "null pointer or stride < 12");
+  int rc = rt_check(rt, nrads);
+  if (rc != SJCS_OK) return rc;
+  sjcs_engine* e = rt->e;
+  rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // the pinned staging is shared with the engine
+  if (rc == SJCS_OK) rc = rt_upload(rt);
+  if (rc != SJCS_OK || n == 0) return rc;
+  const size_t nc = (size_t)n * (size_t)nrads;
+  DevBuf pts, ucounts;
+  std::vector<uint64_t> host;
+  rc = upload_points(e, pts, centrals, stride_bytes, n);
+  if (rc == SJCS_OK) {
+    hipError_t h = ensure(ucounts, nc * sizeof(uint64_t));
+    if (h == hipSuccess) h = hipMemsetAsync(ucounts.p, 0, nc * sizeof(uint64_t), e->stream);
+    if (h != hipSuccess) rc = rt_hip_fail(rt, h, "R-tree count");
+  }
+  if (rc == SJCS_OK)
+    rc = rt_count_into(rt, pts.as<float>(), n, radiuses_sq, nrads, ucounts.as<uint64_t>());
+  if (rc == SJCS_OK) {
+    hipError_t h = hipSuccess;
+    try {
+      host.resize(nc);
+    } catch (const std::bad_alloc&) {
+      h = hipErrorOutOfMemory;
+    }
+    if (h == hipSuccess)
+      h = hipMemcpy(host.data(), ucounts.p, nc * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (h != hipSuccess) rc = rt_hip_fail(rt, h, "R-tree counts to the host");
   }
   if (rc == SJCS_OK)
     for (size_t i = 0; i < nc; ++i) counts[i] += host[i];
   pts.release();
-  r2.release();
-  dcounts.release();
+  ucounts.release();
+  if (rc != SJCS_OK) rt->err = e->err;  // the engine's helpers report there
+  return rc;
+}
+
+int sjcs_rtree_count_device(sjcs_rtree* rt, const float* d_centrals, uint64_t n,
+                            const float* radiuses_sq, int nrads, uint64_t* d_counts) {
+  if (!rt) return SJCS_ERR_INVALID;
+  if ((!d_centrals && n) || (!d_counts && n) || !radiuses_sq)
+    return rt_fail(rt, SJCS_ERR_INVALID, "null pointer");
+  int rc = rt_check(rt, nrads);
+  if (rc != SJCS_OK) return rc;
+  sjcs_engine* e = rt->e;
+  rc = use_device(e);
+  if (rc == SJCS_OK) rc = quiesce(e);  // the sort workspace is shared with the engine
+  if (rc == SJCS_OK) rc = rt_upload(rt);
+  if (rc == SJCS_OK && n) rc = rt_count_into(rt, d_centrals, n, radiuses_sq, nrads, d_counts);
+  if (rc != SJCS_OK) rt->err = e->err;
   return rc;
 }
 

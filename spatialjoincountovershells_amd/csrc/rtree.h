@@ -19,17 +19,19 @@ constexpr int kRtLanes = 16;    // lanes per query: one node's entries side by s
 constexpr int kRtStack = 192;   // nodes pending per query, in LDS
 constexpr int kRtThreads = 256;
 constexpr int kRtQueries = kRtThreads / kRtLanes;  // queries per workgroup
-constexpr uint32_t kRtLeaf = 0x80000000u;           // meta flag of a leaf
+constexpr int kRtRefBits = 28;  // a node reference: node number | entries << 28
+constexpr uint32_t kRtNodeMask = (1u << kRtRefBits) - 1;
 // a halo's count over its (at most 8) images stays below 2^32 in LDS
 constexpr uint64_t kRtMaxPoints = (1ull << 29) - 1;
 constexpr int kRtMaxShells = 1024;
+constexpr uint32_t kXcds = 8;  // MI355X: XCDs that workgroups are dealt to in turn
 
-// The tree on the device.  Node u holds meta[u] = entries | kRtLeaf (for a
-// leaf); its entry k is at u * kRtLanes + k: the low corner of its box (a
-// leaf entry's box is its point), and for an inner node the high corner and
-// the child's node number.  Inner nodes are numbered first.
+// The tree on the device.  Node u's entry k is at u * kRtLanes + k: the low
+// corner of its box (a leaf entry's box is its point), and for an inner node
+// the high corner and a reference to the child: its node number | its entry
+// count << kRtRefBits, so a walk knows how many entries to read without
+// another load.  Inner nodes are numbered first: u >= inner is a leaf.
 struct RtView {
-  const uint32_t* meta;
   const float* lox;
   const float* loy;
   const float* loz;
@@ -49,10 +51,10 @@ struct RtBox {
 // Host copy of that layout, breadth first from the root (every leaf is at
 // depth `height`, so the inner nodes come first).
 struct RtFlat {
-  std::vector<uint32_t> meta;
   std::vector<float> lo[3], hi[3];
   std::vector<uint32_t> child;
   uint32_t inner = 0;
+  uint32_t root = 0;  // the root's reference (node 0)
 };
 
 class RtTree {
@@ -104,12 +106,20 @@ class RtTree {
 };
 
 // counts[h * nrads + k] = rt_rcount's shell-k count of centre h (stored, not
-// added), for n centres (packed xyz, device) against the tree `t` with root
-// `root`.  r2: nrads squared thresholds on the device; sorted: they are
-// non-decreasing (a binary search finds the first match); rbox: the query
-// box half-edge, (float)sqrt(r2[nrads-1]).
-hipError_t rt_count(const RtView& t, uint32_t root, const float* centres, uint64_t n,
-                    const float* r2, int nrads, bool sorted, float rbox, uint64_t* counts,
-                    hipStream_t s);
+// added), for n centres (SoA x, y, z on the device; in Morton order for
+// cache reuse) against the tree `t` whose root has reference `root` and
+// whose first `inner` nodes are inner nodes.  r2: nrads squared
+// thresholds on the device; sorted: they are non-decreasing (a binary search
+// finds the first match); rbox: the query box half-edge,
+// (float)sqrt(r2[nrads-1]).
+hipError_t rt_count(const RtView& t, uint32_t root, uint32_t inner, const float* hx,
+                    const float* hy,
+                    const float* hz, uint64_t n, const float* r2, int nrads, bool sorted,
+                    float rbox, uint32_t* counts, hipStream_t s);
+
+// dst[r * K + k] += c32[inv[r] * K + k] for r < n: counts from the sorted
+// query order back to input rows, added.
+hipError_t rt_add_counts(const uint32_t* c32, const uint32_t* inv, uint64_t* dst, uint64_t n,
+                         int K, hipStream_t s);
 
 }  // namespace sjcs

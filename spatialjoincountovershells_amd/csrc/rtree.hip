@@ -40,7 +40,9 @@
 // One query per 16 lanes: a node's (at most 10) entries are tested side by
 // side, the children to visit are pushed on the query's stack in LDS, a
 // leaf's first zero distance is found by a ballot.  Counts go to per-query
-// LDS bins and are stored once per halo after its last image.
+// LDS bins and are stored once per halo after its last image.  The queries
+// come in Morton order, and each XCD takes one contiguous range of them, so
+// queries that share nodes run close together on one L2.
 #include "rtree.h"
 
 #include <float.h>
@@ -351,7 +353,7 @@ void RtTree::flatten(RtFlat* f) const {
   while (inner < order.size() && !nodes_[order[inner]].leaf) ++inner;
   const size_t m = order.size();
   f->inner = inner;
-  f->meta.assign(m, 0);
+  f->root = (uint32_t)nodes_[root_].n << kRtRefBits;
   for (int d = 0; d < 3; ++d) {
     f->lo[d].assign(m * kRtLanes, 0.0f);
     f->hi[d].assign((size_t)std::max<uint32_t>(inner, 1) * kRtLanes, 0.0f);
@@ -359,13 +361,12 @@ void RtTree::flatten(RtFlat* f) const {
   f->child.assign((size_t)std::max<uint32_t>(inner, 1) * kRtLanes, 0);
   for (size_t i = 0; i < m; ++i) {
     const Node& nd = nodes_[order[i]];
-    f->meta[i] = (uint32_t)nd.n | (nd.leaf ? kRtLeaf : 0u);
     for (int k = 0; k < nd.n; ++k) {
       const size_t at = i * kRtLanes + k;
       for (int d = 0; d < 3; ++d) f->lo[d][at] = nd.box[k].lo[d];
       if (!nd.leaf) {
         for (int d = 0; d < 3; ++d) f->hi[d][at] = nd.box[k].hi[d];
-        f->child[at] = num[nd.ref[k]];
+        f->child[at] = num[nd.ref[k]] | (uint32_t)nodes_[nd.ref[k]].n << kRtRefBits;
       }
     }
   }
@@ -396,13 +397,25 @@ __device__ __forceinline__ int rt_shell(float d, const float* r2, int nrads) {
   }
 }
 
+// Workgroups are dealt to the 8 XCDs in turn; this gives each XCD one
+// contiguous range of the (Morton-ordered) queries, so the nodes that
+// neighbouring queries share are fetched into one L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+  const uint32_t q = nb / kXcds, r = nb % kXcds;
+  return x * q + min(x, r) + i;
+}
+
 template <bool kSorted>
 __global__ __launch_bounds__(kRtThreads) void rt_count_kernel(RtView t, uint32_t root,
-                                                              const float* __restrict__ cen,
+                                                              uint32_t inner,
+                                                              const float* __restrict__ hx,
+                                                              const float* __restrict__ hy,
+                                                              const float* __restrict__ hz,
                                                               uint64_t n,
                                                               const float* __restrict__ r2g,
                                                               int nrads, float rbox,
-                                                              uint64_t* __restrict__ out) {
+                                                              uint32_t* __restrict__ out) {
   extern __shared__ uint32_t rt_lds[];
   const int q = threadIdx.x / kRtLanes;           // query slot of the workgroup
   const int k = threadIdx.x % kRtLanes;           // entry lane
@@ -415,13 +428,15 @@ __global__ __launch_bounds__(kRtThreads) void rt_count_kernel(RtView t, uint32_t
     rt_lds[kRtQueries * kRtStack + i] = 0;
   __syncthreads();
 
-  const uint64_t h = (uint64_t)blockIdx.x * kRtQueries + q;
+  const uint64_t h = (uint64_t)xcd_block() * kRtQueries + q;
   const bool valid = h < n;
   float c[3] = {0.0f, 0.0f, 0.0f}, shift[3] = {0.0f, 0.0f, 0.0f};
   int axes = 0;  // bit d: axis d's query box leaves [0, 1000]
   if (valid) {
+    c[0] = hx[h];
+    c[1] = hy[h];
+    c[2] = hz[h];
     for (int d = 0; d < 3; ++d) {
-      c[d] = cen[3 * h + d];
       if (c[d] - rbox < 0.0f) {
         axes |= 1 << d;
         shift[d] = 1000.0f;
@@ -459,13 +474,12 @@ __global__ __launch_bounds__(kRtThreads) void rt_count_kernel(RtView t, uint32_t
     const bool busy = sp > 0;
     if (__ballot(busy) == 0) break;
     if (busy) {
-      const uint32_t u = stack[sp - 1];
+      const uint32_t ref = stack[sp - 1];
       --sp;
-      const uint32_t m = t.meta[u];
-      const int nch = (int)(m & 0xffu);
-      const bool live = k < nch;
+      const uint32_t u = ref & kRtNodeMask;
+      const bool live = k < (int)(ref >> kRtRefBits);
       const size_t at = (size_t)u * kRtLanes + k;
-      if (m & kRtLeaf) {
+      if (u >= inner) {
         float d = 1.0f;
         if (live) d = dpps_distance(t.lox[at], t.loy[at], t.loz[at], qc[0], qc[1], qc[2]);
         const uint32_t zero = (uint32_t)(__ballot(live && d == 0.0f) >> base) & 0xffffu;
@@ -490,11 +504,33 @@ __global__ __launch_bounds__(kRtThreads) void rt_count_kernel(RtView t, uint32_t
     for (int s = k; s < nrads; s += kRtLanes) out[h * (uint64_t)nrads + s] = bins[s];
 }
 
+__global__ __launch_bounds__(256) void rt_add_counts_kernel(const uint32_t* __restrict__ c32,
+                                                             const uint32_t* __restrict__ inv,
+                                                             uint64_t* __restrict__ dst,
+                                                             uint64_t total, uint32_t K) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * 256) {
+    const uint64_t r = i / K, k = i - r * K;
+    dst[i] += c32[(uint64_t)inv[r] * K + k];
+  }
+}
+
 }  // namespace
 
-hipError_t rt_count(const RtView& t, uint32_t root, const float* centres, uint64_t n,
-                    const float* r2, int nrads, bool sorted, float rbox, uint64_t* counts,
-                    hipStream_t s) {
+hipError_t rt_add_counts(const uint32_t* c32, const uint32_t* inv, uint64_t* dst, uint64_t n,
+                         int K, hipStream_t s) {
+  const uint64_t total = n * (uint64_t)K;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(rt_add_counts_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, c32, inv, dst,
+                     total, (uint32_t)K);
+  return hipGetLastError();
+}
+
+hipError_t rt_count(const RtView& t, uint32_t root, uint32_t inner, const float* hx,
+                    const float* hy, const float* hz, uint64_t n, const float* r2, int nrads,
+                    bool sorted, float rbox, uint32_t* counts, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (nrads < 1 || nrads > kRtMaxShells) return hipErrorInvalidValue;
   const uint64_t blocks = (n + kRtQueries - 1) / kRtQueries;
@@ -502,10 +538,10 @@ hipError_t rt_count(const RtView& t, uint32_t root, const float* centres, uint64
   const size_t lds = sizeof(uint32_t) * ((size_t)kRtQueries * (kRtStack + nrads) + nrads);
   if (sorted)
     hipLaunchKernelGGL(rt_count_kernel<true>, dim3((uint32_t)blocks), dim3(kRtThreads), lds, s, t,
-                       root, centres, n, r2, nrads, rbox, counts);
+                       root, inner, hx, hy, hz, n, r2, nrads, rbox, counts);
   else
     hipLaunchKernelGGL(rt_count_kernel<false>, dim3((uint32_t)blocks), dim3(kRtThreads), lds, s,
-                       t, root, centres, n, r2, nrads, rbox, counts);
+                       t, root, inner, hx, hy, hz, n, r2, nrads, rbox, counts);
   return hipGetLastError();
 }
 

@@ -153,6 +153,30 @@ def test_rt_insert_after_count(built, eng):
     assert np.array_equal(again, second)
 
 
+def test_rt_device_entry_point_adds(built, eng):
+    """sjcs_rtree_count_device: centres and counts on the device, counts
+    added to what they hold (two calls: twice the host entry point's)."""
+    rng = np.random.default_rng(12)
+    p = rng.random((60_000, 3), dtype=np.float32) * np.float32(1000)
+    h = np.vstack([on_particles(rng, p, 700), rng.random((1300, 3)) * 1000]).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.05, 15.0, 24)
+    with built.RTree(eng) as rt:
+        rt.insert(p)
+        want = rt.multi_range_count(h, r2)
+        d_h = eng.device_array(h)
+        d_c = eng.device_array(np.zeros((h.shape[0], r2.size), np.uint64))
+        try:
+            rt.multi_range_count_device(d_h, h.shape[0], r2, d_c)
+            rt.multi_range_count_device(d_h, h.shape[0], r2, d_c)
+            got = np.zeros_like(want)
+            eng._check(eng._lib.sjcs_copy_to_host(eng._h, got.ctypes.data, d_c, got.nbytes))
+        finally:
+            eng.device_free(d_h)
+            eng.device_free(d_c)
+    assert_counts_equal(want, oracle_rt([p], h, r2), h)
+    assert np.array_equal(got, 2 * want)
+
+
 def test_rt_rejects_bad_shell_counts(built, eng):
     with built.RTree(eng) as rt:
         rt.insert(np.zeros((5, 3), np.float32))
