@@ -368,7 +368,9 @@ int sjcs_rtree_create(sjcs_engine *e, sjcs_rtree **out);
  * from the points already inserted.  At most 2^29-1 points per tree.
  * SJCS_ERR_INVALID when the reference's split is undefined for the
  * coordinates (every enlargement difference NaN, where it reads outside its
- * entries); the tree is unusable after a failed insert. */
+ * entries); after a failed insert the tree refuses sjcs_rtree_leaves and
+ * sjcs_rtree_count (SJCS_ERR_STATE), and sjcs_rtree_shape's npoints counts
+ * the points inserted before the failure. */
 int sjcs_rtree_insert(sjcs_rtree *rt, const float *xyz, uint64_t n);
 /* Points, leaves, nodes and height (levels above the leaves). */
 int sjcs_rtree_shape(const sjcs_rtree *rt, uint64_t *npoints, uint64_t *nleaves,

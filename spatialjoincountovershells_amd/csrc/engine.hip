@@ -2565,6 +2565,9 @@ int sjcs_rtree_shape(const sjcs_rtree* rt, uint64_t* npoints, uint64_t* nleaves,
 
 int sjcs_rtree_leaves(const sjcs_rtree* rt, uint64_t* ids, uint32_t* leaf_len) {
   if (!rt || ((!ids || !leaf_len) && rt->tree.points())) return SJCS_ERR_INVALID;
+  // a failed insert can leave entries of a split in the tree that no id
+  // count accounts for
+  if (!rt->tree.ok()) return SJCS_ERR_STATE;
   if (rt->tree.points() == 0) return SJCS_OK;
   rt->tree.leaf_ids(ids, leaf_len);
   return SJCS_OK;
@@ -2578,6 +2581,8 @@ namespace {
 
 int rt_check(sjcs_rtree* rt, int nrads) {
   if (!rt->e) return rt_fail(rt, SJCS_ERR_STATE, "a host-only R-tree cannot count (no engine)");
+  if (!rt->tree.ok())
+    return rt_fail(rt, SJCS_ERR_STATE, "the R-tree is unusable after a failed insert");
   if (nrads < 1 || nrads > kRtMaxShells)
     return rt_fail(rt, SJCS_ERR_INVALID, "R-tree count: 1 to 1024 shells");
   if (9 * rt->tree.height() + 1 > kRtStack)

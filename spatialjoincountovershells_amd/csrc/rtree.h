@@ -64,6 +64,8 @@ class RtTree {
   // held.  False with *err set when the reference's split is undefined for
   // the coordinates (every enlargement difference NaN), or past the limits.
   bool insert(const float* xyz, uint64_t n, std::string* err);
+  // false after a failed insert: the tree may then hold part of a split
+  bool ok() const { return !bad_; }
   uint64_t points() const { return npoints_; }
   int height() const { return level_; }
   uint64_t nodes() const { return nodes_.size(); }

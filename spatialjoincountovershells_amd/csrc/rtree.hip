@@ -307,13 +307,13 @@ bool RtTree::insert(const float* xyz, uint64_t n, std::string* err) {
     return false;
   }
   for (uint64_t i = 0; i < n; ++i) {
-    insert_one(xyz + 3 * i, (uint32_t)(npoints_ + i));
+    insert_one(xyz + 3 * i, (uint32_t)npoints_);
     if (bad_) {
       *err = why_;
       return false;
     }
+    ++npoints_;
   }
-  npoints_ += n;
   return true;
 }
 

@@ -100,8 +100,28 @@ def test_rtree_undefined_split_is_an_error(built):
         with pytest.raises(built.SJCSError) as ei:
             rt.insert(p[10:])
         assert "NaN" in str(ei.value)
+        # the failed tree reports what went in before and refuses the rest
+        assert rt.shape()["points"] == 10
+        with pytest.raises(built.SJCSError) as ei:
+            rt.leaves()
+        assert ei.value.code == built.SJCS_ERR_STATE
     with pytest.raises(ValueError):
         t.OracleRTree(p)
+
+
+def test_failed_insert_mid_batch(built):
+    """A batch whose 11th point forces the undefined split: the points
+    before it count, leaves() refuses (no buffer sized by the count could
+    hold entries the split left behind)."""
+    p = np.vstack([np.full((10, 3), np.nan, np.float32), np.full((5, 3), np.nan, np.float32)])
+    with built.RTree() as rt:
+        with pytest.raises(built.SJCSError):
+            rt.insert(p)
+        assert rt.shape()["points"] == 10
+        with pytest.raises(built.SJCSError):
+            rt.leaves()
+        with pytest.raises(built.SJCSError):
+            rt.insert(p[:1])
 
 
 def test_host_only_tree_cannot_count(built):
