@@ -106,6 +106,23 @@ def test_rt_api_matches_oracle(built, eng, kind):
     assert_counts_equal(got, want, h)
 
 
+def test_rt_million_points(built, eng):
+    """1e6 clustered and uniform points (height-6 tree), 2e4 halos, a
+    quarter on particles and a tenth near faces, 40 shells, against the
+    oracle's R-tree."""
+    rng = np.random.default_rng(17)
+    c = rng.random((2000, 3)) * 1000
+    p = np.vstack([np.mod(c[rng.integers(0, 2000, 600_000)] + rng.normal(0, 3.0, (600_000, 3)),
+                          1000), rng.random((400_000, 3)) * 1000]).astype(np.float32)
+    h = np.vstack([on_particles(rng, p, 5000), rng.random((15_000, 3)) * 1000]).astype(np.float32)
+    h[-2000:, 1] = rng.uniform(0, 4, 2000).astype(np.float32)
+    _, r2 = t.oracle_thresholds(0.001, 5.0, 40)
+    got = rt_count(built, eng, [p], h, r2)
+    want = oracle_rt([p], h, r2)
+    assert want.sum() > 1e5
+    assert_counts_equal(got, want, h)
+
+
 @pytest.mark.parametrize("shells", ["one", "unsorted", "many"])
 def test_rt_shell_sets(built, eng, shells):
     """The first-match rule for any threshold order (linear scan) and the
