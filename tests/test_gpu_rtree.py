@@ -241,3 +241,23 @@ def test_rt_cli_backlog(rt_cli, tmp_path):
     assert out.read_bytes() == t.read_expected_text(case["rt_expected"])
     assert "  -- with backlog execution" in r.stdout
     assert r.stdout.count(">>>> index getting time") == 2
+
+
+@pytest.mark.parametrize("name", ["uni_8", "tdpb_4"])
+@pytest.mark.parametrize("as_rt", [False, True])
+def test_cli_result_on_stdout(built, rt_cli, name, as_rt):
+    """No --output-file: outputResult writes the result to stdout
+    (src/astr_rcount/main.c:383-385, rt_rcount/main.c the same), between
+    the total searching time line and the result writing time line."""
+    case = next(c for c in RT_CASES if c["name"] == name)
+    exe = rt_cli if as_rt else built.CLI_PATH
+    args = [exe, "--thread", "4", "--next-setup", "2", "--particle-files"] + \
+        case["particle_files"] + ["--halo-file", case["halo_file"], "--radius", case["radius"]]
+    r = subprocess.run(args + case["extra"], cwd=t.GOLD, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = case["rt_stdout_masked" if as_rt else "stdout_masked"]
+    tail = ">>>> result writing time: # milli seconds\n"
+    assert lines.endswith(tail)
+    result = t.read_expected_text(case["rt_expected" if as_rt else "expected"]).decode()
+    assert mask(r.stdout) == lines[:-len(tail)] + result + tail
