@@ -185,7 +185,8 @@ def test_cli_backlog_modes(built, backlog, tmp_path):
 def test_dpps_equivalence_on_the_gpu_host():
     """The DPPS probe (the FP32 distance as the reference's SSE4.1 dot
     product rounds it, against the oracle's plain expression) on the host
-    of the GPU box: the CPU whose reference binary the benchmark times."""
+    CPU of the machine with the GPU: the CPU whose reference binary the
+    benchmark times."""
     t.oracle()
     out = subprocess.run([t.DPPS_PROBE, "3000000", "4242"], capture_output=True, text=True,
                          timeout=300)
@@ -1206,7 +1207,7 @@ def test_full_size_config2_properties(eng):
 
 
 def test_full_size_config3_properties(eng):
-    """The north-star scale: 3 particle files of 1e8 uniform particles
+    """The largest uniform workload: 3 particle files of 1e8 uniform particles
     (distinct seeds) x 1e7 halos x 40 shells, counts accumulated on the
     device.  The 3-file totals equal the sum of each file counted alone,
     equal the graph-replayed chunk loop, equal the oracle on a sample of
