@@ -142,7 +142,7 @@ template <int GROUP, int SEG, int kShell, bool kLds, bool kInbox, bool kStage>
 __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     IndexView ix, HaloView hv, Shells sh, uint32_t* __restrict__ counts,
     uint32_t halos_per_block, int accumulate, unsigned long long* __restrict__ inbox_total,
-    uint32_t stage_cap) {
+    uint32_t stage_cap, uint32_t copies, uint32_t ks) {
   constexpr int H = 64 / GROUP;
   // Sorted shells: thresholds are finite or +inf, so a live halo's boxes are
   // NaN-free and a counted pair needs dist < r2max; each row's x run is cut
@@ -152,7 +152,12 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
   extern __shared__ __align__(16) uint32_t smem[];
   const int K = sh.nrads;
   const int kpad = (K + 3) & ~3;
-  // LDS: [lut 2*kLutSize][r2 kpad][marks 4 waves x 128][bins 4 waves x H x K]
+  // LDS: [lut 2*kLutSize][r2 kpad][marks 4 waves x 128][bins 4 waves x H x
+  // copies x ks].  A halo's histogram is kept in `copies` interleaved copies
+  // (a power of two; lane gl adds into copy gl % copies), ks words apart: ks
+  // is odd when copies > 1, so one shell's copies sit in different LDS
+  // banks.  With 64 lanes on one halo the hits of a round crowd into its
+  // outer shells, and one copy would take them a lane at a time.
   ShellEntry* s_lut = reinterpret_cast<ShellEntry*>(smem);
   float* s_r2 = reinterpret_cast<float*>(smem + 2 * kLutSize);
   uint32_t* s_mark = kLds ? smem + 2 * kLutSize + kpad : smem;
@@ -172,8 +177,15 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
   const int grp = lane / GROUP;
   const int gl = lane % GROUP;
   uint32_t* mark = s_mark + wave * 128 + grp * 2 * GROUP;
-  uint32_t* bins = s_bins + (wave * H + grp) * K;
-  char* bins_bytes = reinterpret_cast<char*>(bins);
+  // one copy below 64 lanes per halo: constants, so those kernels keep
+  // their plain histogram code
+  if constexpr (GROUP != 64) {
+    copies = 1u;
+    ks = (uint32_t)K;
+  }
+  const uint32_t nbins = copies * ks;
+  uint32_t* bins = s_bins + (wave * H + grp) * nbins;
+  char* bins_bytes = reinterpret_cast<char*>(bins + (gl & (copies - 1)) * ks);
 
   const __amdgpu_buffer_rsrc_t rx = make_rsrc(ix.x, ix.n);
   const __amdgpu_buffer_rsrc_t ry = make_rsrc(ix.y, ix.n);
@@ -208,7 +220,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
   // LDS after the histograms: [s_delta kStageRows][row start, length,
   // slot: 3 x kStageRows][union bounds and scan: 32][x, y, z: 3 x
   // (stage_cap + 8)]
-  uint32_t* s_delta = s_bins + kCountWaves * H * K;
+  uint32_t* s_delta = s_bins + kCountWaves * H * nbins;
   float* s_px = reinterpret_cast<float*>(s_delta + 4 * kStageRows + 32);
   float* s_py = s_px + stage_cap + 8;
   float* s_pz = s_py + stage_cap + 8;
@@ -321,7 +333,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       if (hv.slot) row = hv.slot[hs];
     }
     if (kLds && !kInbox) {
-      for (int k = gl; k < K; k += GROUP) bins[k] = 0u;
+      for (int i = gl; i < (int)nbins; i += GROUP) bins[i] = 0u;
     }
     // A NaN centre makes every distance NaN: nothing is ever counted.  Its
     // box still passes every coordinate on the NaN axes, and the in-box
@@ -421,47 +433,96 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         const uint32_t stotal = grp_lane<GROUP>(sincl, GROUP - 1);
         const uint32_t max_stotal = wave_max_of_groups<GROUP>(stotal);
         uint32_t carry = 0;
-        for (uint32_t c0 = 0; c0 < max_stotal; c0 += GROUP) {
-          // rows whose first segment falls in [c0, c0 + GROUP) mark its
-          // slot; a max-scan over the marks gives every slot its row
+        // 64 lanes per halo: the next round's candidates are loaded before
+        // this round's are tested (its loads wait out the tests).  Fewer
+        // lanes per halo keep one round in flight: the second set of
+        // registers costs them more occupancy than the overlap gains.
+        constexpr bool kAhead = GROUP == 64;
+        // Round c0's candidates of this lane (64 lanes: never staged), as in
+        // the round code below, for the round ahead.  Called by the whole
+        // wave, rounds in order (carry).
+        auto fetch_ahead = [&](uint32_t c0, float (&fx)[SEG], float (&fy)[SEG],
+                               float (&fz)[SEG], uint32_t& slen) {
           mark[gl] = 0u;
           wave_sync();
           if (nseg > 0 && sexcl >= c0 && sexcl < c0 + GROUP) mark[sexcl - c0] = (uint32_t)gl + 1;
           wave_sync();
           const uint32_t m = grp_scan_max<GROUP>(mark[gl], gl);
           const uint32_t j = m ? m - 1 : carry;
-          // cross-lane reads run on every lane: one taken inside a branch
-          // would read 0 from lanes the branch left inactive
           carry = grp_lane<GROUP>(j, GROUP - 1);
           const uint32_t rbeg = grp_lane<GROUP>(beg, j);
           const uint32_t rlen = grp_lane<GROUP>(len, j);
           const uint32_t rsx = grp_lane<GROUP>(sexcl, j);
           const uint32_t q = c0 + (uint32_t)gl;
-          const uint32_t off = (q - rsx) * SEG;  // segment start within the row
-          const uint32_t slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+          const uint32_t off = (q - rsx) * SEG;
+          slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+          wave_sync();  // the next call rewrites the marks
+          const uint32_t boff = (rbeg + off) * 4u;
+#pragma unroll
+          for (int u = 0; u < SEG; ++u) {
+            fx[u] = buf_load(rx, boff + 4u * u);
+            fy[u] = buf_load(ry, boff + 4u * u);
+            fz[u] = buf_load(rz, boff + 4u * u);
+          }
+        };
+        float nx[SEG], ny[SEG], nz[SEG];
+        uint32_t nslen = 0;
+        if constexpr (kAhead) {
+          if (max_stotal > 0) fetch_ahead(0, nx, ny, nz, nslen);
+        }
+        for (uint32_t c0 = 0; c0 < max_stotal; c0 += GROUP) {
           float px[SEG], py[SEG], pz[SEG];
-          if (from_lds) {
-            // the staged row's slots; a slot past its group's segments reads
-            // slot 0, and one past its row's end at most SEG - 1 beyond the
-            // last row (padded): values it never uses
-            const uint32_t rsd = grp_lane<GROUP>(sdelta, j);
-            const uint32_t li = q < stotal ? rbeg + off + rsd : 0u;
+          uint32_t slen;
+          if constexpr (kAhead) {
 #pragma unroll
             for (int u = 0; u < SEG; ++u) {
-              px[u] = s_px[li + u];
-              py[u] = s_py[li + u];
-              pz[u] = s_pz[li + u];
+              px[u] = nx[u];
+              py[u] = ny[u];
+              pz[u] = nz[u];
             }
+            slen = nslen;
+            if (c0 + GROUP < max_stotal) fetch_ahead(c0 + GROUP, nx, ny, nz, nslen);
           } else {
-            // a slot past its group's segments loads whatever its offset
-            // names (bounds-checked by the buffer descriptor) and never
-            // uses it
-            const uint32_t boff = (rbeg + off) * 4u;
+            // rows whose first segment falls in [c0, c0 + GROUP) mark its
+            // slot; a max-scan over the marks gives every slot its row
+            mark[gl] = 0u;
+            wave_sync();
+            if (nseg > 0 && sexcl >= c0 && sexcl < c0 + GROUP) mark[sexcl - c0] = (uint32_t)gl + 1;
+            wave_sync();
+            const uint32_t m = grp_scan_max<GROUP>(mark[gl], gl);
+            const uint32_t j = m ? m - 1 : carry;
+            // cross-lane reads run on every lane: one taken inside a branch
+            // would read 0 from lanes the branch left inactive
+            carry = grp_lane<GROUP>(j, GROUP - 1);
+            const uint32_t rbeg = grp_lane<GROUP>(beg, j);
+            const uint32_t rlen = grp_lane<GROUP>(len, j);
+            const uint32_t rsx = grp_lane<GROUP>(sexcl, j);
+            const uint32_t q = c0 + (uint32_t)gl;
+            const uint32_t off = (q - rsx) * SEG;  // segment start within the row
+            slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+            if (from_lds) {
+              // the staged row's slots; a slot past its group's segments
+              // reads slot 0, and one past its row's end at most SEG - 1
+              // beyond the last row (padded): values it never uses
+              const uint32_t rsd = grp_lane<GROUP>(sdelta, j);
+              const uint32_t li = q < stotal ? rbeg + off + rsd : 0u;
 #pragma unroll
-            for (int u = 0; u < SEG; ++u) {
-              px[u] = buf_load(rx, boff + 4u * u);
-              py[u] = buf_load(ry, boff + 4u * u);
-              pz[u] = buf_load(rz, boff + 4u * u);
+              for (int u = 0; u < SEG; ++u) {
+                px[u] = s_px[li + u];
+                py[u] = s_py[li + u];
+                pz[u] = s_pz[li + u];
+              }
+            } else {
+              // a slot past its group's segments loads whatever its offset
+              // names (bounds-checked by the buffer descriptor) and never
+              // uses it
+              const uint32_t boff = (rbeg + off) * 4u;
+#pragma unroll
+              for (int u = 0; u < SEG; ++u) {
+                px[u] = buf_load(rx, boff + 4u * u);
+                py[u] = buf_load(ry, boff + 4u * u);
+                pz[u] = buf_load(rz, boff + 4u * u);
+              }
             }
           }
           if constexpr (kSphere) {
@@ -535,7 +596,8 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       if (active) {
         uint32_t* dst = counts + (uint64_t)row * K;
         for (int k = gl; k < K; k += GROUP) {
-          const uint32_t v = bins[k];
+          uint32_t v = 0;
+          for (uint32_t c = 0; c < copies; ++c) v += bins[c * ks + k];
           if (accumulate) dst[k] += v; else dst[k] = v;
         }
       }
@@ -559,18 +621,33 @@ struct CountLaunch {
   uint32_t stage_cap;  // 0: no particle staging
 };
 
+// Copies of a halo's LDS histogram: 8 with 64 lanes per halo (as many as
+// fit kMaxGroupBinBytes), else 1; and the words between copies.
+int bin_copies(int nrads, int group) {
+  if (nrads > kMaxLdsShells || group != 64) return 1;
+  int c = 8;
+  while (c > 1 && (size_t)kCountWaves * c * (size_t)(nrads | 1) * 4 > kMaxGroupBinBytes) c >>= 1;
+  return c;
+}
+
+uint32_t bin_stride(int nrads, int group) {
+  return bin_copies(nrads, group) > 1 ? (uint32_t)(nrads | 1) : (uint32_t)nrads;
+}
+
 template <int GROUP, int kShell, bool kLds, bool kInbox, bool kStage>
 void launch_group(const IndexView& ix, const HaloView& hv, const Shells& sh, uint32_t* counts,
                   int accumulate, unsigned long long* inbox, const CountLaunch& L,
                   hipStream_t s) {
+  const uint32_t copies = (uint32_t)bin_copies(sh.nrads, GROUP);
+  const uint32_t ks = bin_stride(sh.nrads, GROUP);
   if (L.seg >= 8)
     count_group_kernel<GROUP, 8, kShell, kLds, kInbox, kStage>
         <<<L.nblocks, kCountThreads, L.lds, s>>>(ix, hv, sh, counts, L.halos_per_block,
-                                                 accumulate, inbox, L.stage_cap);
+                                                 accumulate, inbox, L.stage_cap, copies, ks);
   else
     count_group_kernel<GROUP, 4, kShell, kLds, kInbox, kStage>
         <<<L.nblocks, kCountThreads, L.lds, s>>>(ix, hv, sh, counts, L.halos_per_block,
-                                                 accumulate, inbox, L.stage_cap);
+                                                 accumulate, inbox, L.stage_cap, copies, ks);
 }
 
 template <int GROUP, int kShell>
@@ -705,7 +782,8 @@ size_t count_lds_bytes(int nrads, int group) {
   if (nrads > kMaxLdsShells) return kNoLdsBytes;
   const size_t kpad = (size_t)((nrads + 3) & ~3);
   const size_t h = (size_t)(64 / group);
-  return 4 * (2 * (size_t)kLutSize + kpad + kCountWaves * 128 + kCountWaves * h * nrads);
+  const size_t bins = (size_t)bin_copies(nrads, group) * bin_stride(nrads, group);
+  return 4 * (2 * (size_t)kLutSize + kpad + kCountWaves * 128 + kCountWaves * h * bins);
 }
 
 hipError_t count_halos(const IndexView& ix, const HaloView& hv, const Shells& sh,
