@@ -107,12 +107,27 @@ __device__ __forceinline__ uint32_t grp_lane(uint32_t v, uint32_t j) {
   return (uint32_t)__shfl((int)v, (int)j, GROUP);
 }
 
-// Max over the wave of a value that is uniform within each group.
+// Value of the last lane of this lane's group (an inclusive scan's total):
+// 16 lanes by a DPP row_newbcast:15, 64 lanes by a lane read.
+template <int GROUP>
+__device__ __forceinline__ uint32_t grp_last(uint32_t v) {
+  if constexpr (GROUP == 16)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x15F, 0xf, 0xf, false);
+  else if constexpr (GROUP == 64)
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+  else
+    return grp_lane<GROUP>(v, GROUP - 1);
+}
+
+// Max over the wave of a value that is uniform within each group, read from
+// each group's first lane (a scalar: loops bounded by it stay uniform).
 template <int GROUP>
 __device__ __forceinline__ uint32_t wave_max_of_groups(uint32_t v) {
+  uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
 #pragma unroll
-  for (int off = GROUP; off < 64; off <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
-  return v;
+  for (int g = GROUP; g < 64; g += GROUP)
+    m = max(m, (uint32_t)__builtin_amdgcn_readlane((int)v, g));
+  return m;
 }
 
 // Inclusive sum over a workgroup of kCountThreads, one value per thread
@@ -318,6 +333,10 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
     }
   }
 
+  // the group's histogram starts at zero; each flush leaves it zero again
+  if (kLds && !kInbox) {
+    for (int i = gl; i < (int)nbins; i += GROUP) bins[i] = 0u;
+  }
   for (uint64_t hb = h0 + (uint64_t)wave * H; hb < h1; hb += (uint64_t)kCountWaves * H) {
     const uint64_t hs = hb + grp;
     const bool active = hs < h1;
@@ -332,9 +351,6 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       cz = hv.z[hs];
       if (hv.slot) row = hv.slot[hs];
     }
-    if (kLds && !kInbox) {
-      for (int i = gl; i < (int)nbins; i += GROUP) bins[i] = 0u;
-    }
     // A NaN centre makes every distance NaN: nothing is ever counted.  Its
     // box still passes every coordinate on the NaN axes, and the in-box
     // measurement counts those pair tests as the reference makes them.
@@ -348,14 +364,10 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
       const float c[3] = {cx, cy, cz};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const float lo = c[d] - rbox, hi = c[d] + rbox;
-        if (lo < kBoxLo) {
-          shift[d] = kBoxHi;
-          oob |= 1 << d;
-        } else if (hi > kBoxHi) {
-          shift[d] = -kBoxHi;
-          oob |= 1 << d;
-        }
+        const bool below = c[d] - rbox < kBoxLo;
+        const bool above = c[d] + rbox > kBoxHi;
+        shift[d] = below ? kBoxHi : (above ? -kBoxHi : 0.0f);
+        oob |= (below || above) ? 1 << d : 0;
       }
     }
     // images any halo of this wavefront has (scalar: skipping costs nothing)
@@ -430,7 +442,11 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         const uint32_t nseg = (len + SEG - 1) / SEG;
         const uint32_t sincl = grp_scan_add<GROUP>(nseg, gl);
         const uint32_t sexcl = sincl - nseg;
-        const uint32_t stotal = grp_lane<GROUP>(sincl, GROUP - 1);
+        const uint32_t stotal = grp_last<GROUP>(sincl);
+        // segment q of row j starts at particle abase_j + q * SEG and its row
+        // ends at aend_j (abase wraps modulo 2^32; the sum does not)
+        const uint32_t abase = beg - sexcl * (uint32_t)SEG;
+        const uint32_t aend = beg + len;
         const uint32_t max_stotal = wave_max_of_groups<GROUP>(stotal);
         uint32_t carry = 0;
         // 64 lanes per halo: the next round's candidates are loaded before
@@ -449,15 +465,13 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
           wave_sync();
           const uint32_t m = grp_scan_max<GROUP>(mark[gl], gl);
           const uint32_t j = m ? m - 1 : carry;
-          carry = grp_lane<GROUP>(j, GROUP - 1);
-          const uint32_t rbeg = grp_lane<GROUP>(beg, j);
-          const uint32_t rlen = grp_lane<GROUP>(len, j);
-          const uint32_t rsx = grp_lane<GROUP>(sexcl, j);
+          carry = grp_last<GROUP>(j);
           const uint32_t q = c0 + (uint32_t)gl;
-          const uint32_t off = (q - rsx) * SEG;
-          slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+          const uint32_t start = grp_lane<GROUP>(abase, j) + q * (uint32_t)SEG;
+          const uint32_t rend = grp_lane<GROUP>(aend, j);
+          slen = q < stotal ? min((uint32_t)SEG, rend - start) : 0u;
           wave_sync();  // the next call rewrites the marks
-          const uint32_t boff = (rbeg + off) * 4u;
+          const uint32_t boff = start * 4u;
 #pragma unroll
           for (int u = 0; u < SEG; ++u) {
             fx[u] = buf_load(rx, boff + 4u * u);
@@ -493,19 +507,17 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
             const uint32_t j = m ? m - 1 : carry;
             // cross-lane reads run on every lane: one taken inside a branch
             // would read 0 from lanes the branch left inactive
-            carry = grp_lane<GROUP>(j, GROUP - 1);
-            const uint32_t rbeg = grp_lane<GROUP>(beg, j);
-            const uint32_t rlen = grp_lane<GROUP>(len, j);
-            const uint32_t rsx = grp_lane<GROUP>(sexcl, j);
+            carry = grp_last<GROUP>(j);
             const uint32_t q = c0 + (uint32_t)gl;
-            const uint32_t off = (q - rsx) * SEG;  // segment start within the row
-            slen = q < stotal ? min((uint32_t)SEG, rlen - off) : 0u;
+            const uint32_t start = grp_lane<GROUP>(abase, j) + q * (uint32_t)SEG;
+            const uint32_t rend = grp_lane<GROUP>(aend, j);
+            slen = q < stotal ? min((uint32_t)SEG, rend - start) : 0u;
             if (from_lds) {
               // the staged row's slots; a slot past its group's segments
               // reads slot 0, and one past its row's end at most SEG - 1
               // beyond the last row (padded): values it never uses
               const uint32_t rsd = grp_lane<GROUP>(sdelta, j);
-              const uint32_t li = q < stotal ? rbeg + off + rsd : 0u;
+              const uint32_t li = q < stotal ? start + rsd : 0u;
 #pragma unroll
               for (int u = 0; u < SEG; ++u) {
                 px[u] = s_px[li + u];
@@ -516,7 +528,7 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
               // a slot past its group's segments loads whatever its offset
               // names (bounds-checked by the buffer descriptor) and never
               // uses it
-              const uint32_t boff = (rbeg + off) * 4u;
+              const uint32_t boff = start * 4u;
 #pragma unroll
               for (int u = 0; u < SEG; ++u) {
                 px[u] = buf_load(rx, boff + 4u * u);
@@ -597,7 +609,10 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         uint32_t* dst = counts + (uint64_t)row * K;
         for (int k = gl; k < K; k += GROUP) {
           uint32_t v = 0;
-          for (uint32_t c = 0; c < copies; ++c) v += bins[c * ks + k];
+          for (uint32_t c = 0; c < copies; ++c) {
+            v += bins[c * ks + k];
+            bins[c * ks + k] = 0u;
+          }
           if (accumulate) dst[k] += v; else dst[k] = v;
         }
       }
