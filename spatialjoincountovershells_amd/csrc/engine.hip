@@ -288,8 +288,9 @@ struct sjcs_engine {
   DevBuf in;  // halo / query staging on the device
   DevBuf ws;  // sort / scan workspace
   DevBuf inbox;
-  void* pinned = nullptr;  // pinned staging of halos / queries
+  void* pinned = nullptr;  // pinned staging of halos / queries, and of count downloads
   size_t pinned_bytes = 0;
+  hipEvent_t dl[2] = {nullptr, nullptr};  // a download piece reached page-locked memory
 
   int64_t cell_size_milli = 0;
   int64_t kernel = 0;
@@ -382,6 +383,56 @@ int ensure_pinned(sjcs_engine* e, size_t bytes) {
   e->pinned_bytes = 0;
   HIPCALL(e, hipHostMalloc(&e->pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
   e->pinned_bytes = bytes;
+  return SJCS_OK;
+}
+
+// Host threads moving one page-locked piece into the caller's memory.
+void parallel_copy(char* dst, const char* src, size_t bytes) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t nt = std::min<size_t>(std::max(hw, 1u), 16u);
+  const size_t part = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
+  std::vector<std::thread> ts;
+  for (size_t t = 1; t < nt && t * part < bytes; ++t)
+    ts.emplace_back([=] { memcpy(dst + t * part, src + t * part, std::min(part, bytes - t * part)); });
+  memcpy(dst, src, std::min(part, bytes));
+  for (std::thread& t : ts) t.join();
+}
+
+// Device memory -> the caller's (pageable) host memory.  A large copy goes
+// in pieces through two page-locked buffers: piece i + 1 crosses the link
+// while host threads move piece i into place (a copy straight into pageable
+// memory runs at a fraction of the link's rate).  The engine is idle when
+// this is called.
+constexpr size_t kDownloadPiece = 64ull << 20;
+int download(sjcs_engine* e, void* dst, const void* d_src, size_t bytes) {
+  if (bytes <= kDownloadPiece) {
+    HIPCALL(e, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, e->stream),
+            "hipMemcpy(counts)");
+    HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return SJCS_OK;
+  }
+  int rc = ensure_pinned(e, 2 * kDownloadPiece);
+  if (rc != SJCS_OK) return rc;
+  char* pin = static_cast<char*>(e->pinned);
+  const char* src = static_cast<const char*>(d_src);
+  char* out = static_cast<char*>(dst);
+  const size_t npieces = (bytes + kDownloadPiece - 1) / kDownloadPiece;
+  for (size_t i = 0; i <= npieces; ++i) {
+    if (i < npieces) {
+      // buffer i % 2 held piece i - 2, moved out in the previous iteration
+      const size_t off = i * kDownloadPiece, len = std::min(kDownloadPiece, bytes - off);
+      HIPCALL(e, hipMemcpyAsync(pin + (i % 2) * kDownloadPiece, src + off, len,
+                                hipMemcpyDeviceToHost, e->stream),
+              "hipMemcpy(counts)");
+      HIPCALL(e, hipEventRecord(e->dl[i % 2], e->stream), "hipEventRecord");
+    }
+    if (i >= 1) {
+      const size_t j = i - 1, off = j * kDownloadPiece;
+      HIPCALL(e, hipEventSynchronize(e->dl[j % 2]), "hipEventSynchronize");
+      parallel_copy(out + off, pin + (j % 2) * kDownloadPiece,
+                    std::min(kDownloadPiece, bytes - off));
+    }
+  }
   return SJCS_OK;
 }
 
@@ -1110,6 +1161,7 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
     if (h == hipSuccess) h = hipEventCreateWithFlags(&e->counted[i], hipEventDisableTiming);
     if (h == hipSuccess) h = hipEventCreateWithFlags(&e->uploaded[i], hipEventDisableTiming);
     if (h == hipSuccess) h = hipEventCreateWithFlags(&e->stage_read[i], hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->dl[i], hipEventDisableTiming);
     // recorded once so that the first waits on them return at once
     if (h == hipSuccess) h = hipEventRecord(e->built[i], e->stream);
     if (h == hipSuccess) h = hipEventRecord(e->counted[i], e->stream);
@@ -1149,6 +1201,7 @@ void sjcs_engine_destroy(sjcs_engine* e) {
     if (e->counted[i]) (void)hipEventDestroy(e->counted[i]);
     if (e->uploaded[i]) (void)hipEventDestroy(e->uploaded[i]);
     if (e->stage_read[i]) (void)hipEventDestroy(e->stage_read[i]);
+    if (e->dl[i]) (void)hipEventDestroy(e->dl[i]);
   }
   e->in.release();
   e->ws.release();
@@ -1339,11 +1392,7 @@ int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint6
     e->readout_timed = true;
     e->readout_gen = e->counts_gen;
   }
-  HIPCALL(e, hipMemcpyAsync(counts, e->ucounts.as<uint64_t>() + first * K, n * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost, e->stream),
-          "hipMemcpy(counts)");
-  HIPCALL(e, hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-  return SJCS_OK;
+  return download(e, counts, e->ucounts.as<uint64_t>() + first * K, n * sizeof(uint64_t));
 }
 
 int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {
