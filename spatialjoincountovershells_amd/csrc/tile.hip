@@ -8,9 +8,10 @@
 //
 // For each image the wavefront takes the union of the boxes of the tile's
 // halos that have that image, cuts every cell row of it to the union of their
-// spheres, and deals the rows' particles out 64 at a time (lane = candidate,
-// found by a binary search over the rows' prefix sums).  Each candidate is
-// then tested against every halo of the tile in turn, with the halo's image
+// spheres, and deals the rows' particles out in segments of four
+// consecutive particles, 64 segments at a time (lane = segment, its row
+// found by a binary search over the rows' segment prefix sums).  Each
+// candidate is then tested against every halo of the tile in turn, with the halo's image
 // centre, safe radius and box read from LDS by broadcast; hits go to the
 // halo's LDS histogram, flushed to the counts (rows of consecutive halos are
 // contiguous) when the tile is done.  Halos of a tile at the very same
@@ -21,7 +22,7 @@
 // Tiles differ in cost by orders of magnitude (a halo at a big clump sees
 // 1e5 particles more than one in the field).  A pre-pass estimates every
 // tile's pair tests and cuts heavy tiles into parts (a part takes every
-// nparts-th chunk of 64 candidates and adds its histograms atomically); the
+// nparts-th round of 64 segments and adds its histograms atomically); the
 // count is a persistent kernel whose wavefronts take work items (tile, part)
 // from a queue until it is empty.
 #include "tile.h"
@@ -37,7 +38,8 @@ namespace sjcs {
 namespace {
 
 constexpr int kHaloWords = 12;  // sx sy sz safe lx hx ly hy lz hz mult (padded)
-constexpr int kWaveWords = 64 * kHaloWords + 128 + 64 + 64 + kTileBinWords;
+constexpr int kWaveWords = 64 * kHaloWords + 192 + 64 + 64 + kTileBinWords;
+constexpr int kTileSeg = 4;     // consecutive particles a lane takes per round
 constexpr uint64_t kItemPairs = 1ull << 22;  // pair tests a work item aims at
 constexpr uint32_t kMaxParts = 256;
 constexpr uint32_t kTileGrid = 3072;         // persistent wavefronts
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
   extern __shared__ __align__(16) uint32_t smem[];
   const int K = sh.nrads;
   const int kpad = (K + 3) & ~3;
-  // LDS: [r2 kpad][halos 64*kHaloWords][rows 64*2][valid-halo list 64]
+  // LDS: [r2 kpad][halos 64*kHaloWords][rows 64*3][valid-halo list 64]
   // [representatives 64][bins kTileBinWords].  The shell table stays in
   // global memory (L1-resident): a copy per wavefront would cost 8 KB of LDS
   // and about half of the wavefronts a CU can hold.
@@ -132,8 +134,10 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
   const int lane = threadIdx.x & 63;
   uint32_t* wbase = smem + kpad;
   float* hd = reinterpret_cast<float*>(wbase);
-  uint32_t* rows = wbase + 64 * kHaloWords;  // [64] first particle, [64] prefix
-  uint32_t* hl = rows + 128;                 // the image's valid halos
+  // per row j of a batch: [64] base (segment q starts at base + q * kTileSeg),
+  // [64] first segment, [64] end particle
+  uint32_t* rows = wbase + 64 * kHaloWords;
+  uint32_t* hl = rows + 192;                 // the image's valid halos
   uint32_t* s_rep = hl + 64;                 // first halo of the tile at the same spot
   uint32_t* bins = s_rep + 64;
   const uint32_t nitems = item_start[tv.n];
@@ -284,27 +288,41 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
             len = ix.cell_end[k1] - beg;
           }
         }
-        const uint32_t incl = wave_incl_sum(len, lane);
-        const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
-        rows[lane] = beg;
-        rows[64 + lane] = incl - len;
+        // rows cut into segments of kTileSeg consecutive particles, dealt
+        // out 64 per round (a part takes every np-th round)
+        const uint32_t nseg = (len + kTileSeg - 1) / kTileSeg;
+        const uint32_t sincl = wave_incl_sum(nseg, lane);
+        const uint32_t stotal = (uint32_t)__builtin_amdgcn_readlane((int)sincl, 63);
+        const uint32_t sexcl = sincl - nseg;
+        rows[lane] = beg - sexcl * (uint32_t)kTileSeg;  // wraps modulo 2^32; the sums do not
+        rows[64 + lane] = sexcl;
+        rows[128 + lane] = beg + len;
         wave_sync();
-        for (uint32_t c0 = p * 64u; c0 < total; c0 += np * 64u) {
+        for (uint32_t c0 = p * 64u; c0 < stotal; c0 += np * 64u) {
           const uint32_t q = c0 + (uint32_t)lane;
-          const bool inq = q < total;
-          // the last row whose prefix is <= q holds candidate q
+          // the last row whose first segment is <= q holds segment q (an
+          // empty row shares its first segment with the row after it)
           int j = 0;
 #pragma unroll
           for (int step = 32; step > 0; step >>= 1)
             if (rows[64 + j + step] <= q) j += step;
-          const uint32_t idx = rows[j] + (q - rows[64 + j]);
-          const uint32_t boff = inq ? idx * 4u : 0xfffffffcu;  // out of range: loads 0
-          const float px = buf_load(rx, boff), py = buf_load(ry, boff), pz = buf_load(rz, boff);
+          const uint32_t start = rows[j] + q * (uint32_t)kTileSeg;
+          const uint32_t slen = q < stotal ? min((uint32_t)kTileSeg, rows[128 + j] - start) : 0u;
+          // a slot past the segments loads whatever its offset names
+          // (bounds-checked by the buffer descriptor) and never uses it
+          const uint32_t boff = start * 4u;
+          float px[kTileSeg], py[kTileSeg], pz[kTileSeg];
+#pragma unroll
+          for (int u = 0; u < kTileSeg; ++u) {
+            px[u] = buf_load(rx, boff + 4u * u);
+            py[u] = buf_load(ry, boff + 4u * u);
+            pz[u] = buf_load(rz, boff + 4u * u);
+          }
           // the valid halos, NB at a time (8 while at least 8 are left, then
           // 4, 2, 1: nv is the same on every lane, and a tile of copies of
           // one position has a single valid halo): their LDS reads go out
-          // together, then the distances, then the shell-table reads, then
-          // the bin increments
+          // together once per round, then for each of the lane's candidates
+          // the distances, the shell-table reads and the bin increments
           auto batch = [&](auto nbc, int hb) {
             constexpr int NB = decltype(nbc)::value;
             int hh[NB];
@@ -314,42 +332,46 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
 #pragma unroll
             for (int u = 0; u < NB; ++u)
               c[u] = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords);
-            if constexpr (kInbox) {
 #pragma unroll
-              for (int u = 0; u < NB; ++u) {
-                const float4 b0 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 4);
-                const float4 b1 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 8);
-                const bool in = inq && !(px < b0.x) && !(b0.y < px) && !(py < b0.z) &&
-                                !(b0.w < py) && !(pz < b1.x) && !(b1.y < pz);
-                inbox += in ? __builtin_bit_cast(uint32_t, b1.z) : 0u;
-              }
-            } else {
-              float dist[NB];
-              bool hit[NB];
+            for (int k = 0; k < kTileSeg; ++k) {
+              const bool inq = (uint32_t)k < slen;
+              if constexpr (kInbox) {
 #pragma unroll
-              for (int u = 0; u < NB; ++u) {
-                dist[u] = dpps_distance(px, py, pz, c[u].x, c[u].y, c[u].z);
-                hit[u] = inq & (dist[u] < sh.r2max);
-                if (hit[u] && !(dist[u] < c[u].w)) {
+                for (int u = 0; u < NB; ++u) {
                   const float4 b0 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 4);
-                  const float2 b1 = *reinterpret_cast<const float2*>(hd + hh[u] * kHaloWords + 8);
-                  hit[u] = !(px < b0.x) && !(b0.y < px) && !(py < b0.z) && !(b0.w < py) &&
-                           !(pz < b1.x) && !(b1.y < pz);
+                  const float4 b1 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 8);
+                  const bool in = inq && !(px[k] < b0.x) && !(b0.y < px[k]) && !(py[k] < b0.z) &&
+                                  !(b0.w < py[k]) && !(pz[k] < b1.x) && !(b1.y < pz[k]);
+                  inbox += in ? __builtin_bit_cast(uint32_t, b1.z) : 0u;
                 }
-              }
-              ShellEntry e[NB];
+              } else {
+                float dist[NB];
+                bool hit[NB];
 #pragma unroll
-              for (int u = 0; u < NB; ++u) {
-                const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
-                e[u] = g_lut[hit[u] ? max(b, 0) : 0];
-              }
+                for (int u = 0; u < NB; ++u) {
+                  dist[u] = dpps_distance(px[k], py[k], pz[k], c[u].x, c[u].y, c[u].z);
+                  hit[u] = inq & (dist[u] < sh.r2max);
+                  if (hit[u] && !(dist[u] < c[u].w)) {
+                    const float4 b0 = *reinterpret_cast<const float4*>(hd + hh[u] * kHaloWords + 4);
+                    const float2 b1 = *reinterpret_cast<const float2*>(hd + hh[u] * kHaloWords + 8);
+                    hit[u] = !(px[k] < b0.x) && !(b0.y < px[k]) && !(py[k] < b0.z) &&
+                             !(b0.w < py[k]) && !(pz[k] < b1.x) && !(b1.y < pz[k]);
+                  }
+                }
+                ShellEntry e[NB];
 #pragma unroll
-              for (int u = 0; u < NB; ++u) {
-                if (hit[u]) {
-                  int o = (int)e[u].off + (dist[u] < e[u].t ? 0 : 4);
-                  if constexpr (kShell == kShellTable)
-                    while (!(dist[u] < s_r2[o >> 2])) o += 4;  // ends: dist < r2[nrads-1]
-                  atomicAdd(bins + hh[u] * K + (o >> 2), 1u);
+                for (int u = 0; u < NB; ++u) {
+                  const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
+                  e[u] = g_lut[hit[u] ? max(b, 0) : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                  if (hit[u]) {
+                    int o = (int)e[u].off + (dist[u] < e[u].t ? 0 : 4);
+                    if constexpr (kShell == kShellTable)
+                      while (!(dist[u] < s_r2[o >> 2])) o += 4;  // ends: dist < r2[nrads-1]
+                    atomicAdd(bins + hh[u] * K + (o >> 2), 1u);
+                  }
                 }
               }
             }
