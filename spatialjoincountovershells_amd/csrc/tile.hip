@@ -8,7 +8,7 @@
 //
 // For each image the wavefront takes the union of the boxes of the tile's
 // halos that have that image, cuts every cell row of it to the union of their
-// spheres, and deals the rows' particles out in segments of four
+// spheres, and deals the rows' particles out in segments of eight
 // consecutive particles, 64 segments at a time (lane = segment, its row
 // found by a binary search over the rows' segment prefix sums).  Each
 // candidate is then tested against every halo of the tile in turn, with the halo's image
@@ -39,7 +39,7 @@ namespace {
 
 constexpr int kHaloWords = 12;  // sx sy sz safe lx hx ly hy lz hz mult (padded)
 constexpr int kWaveWords = 64 * kHaloWords + 192 + 64 + 64 + kTileBinWords;
-constexpr int kTileSeg = 4;     // consecutive particles a lane takes per round
+constexpr int kTileSeg = 8;     // consecutive particles a lane takes per round
 constexpr uint64_t kItemPairs = 1ull << 22;  // pair tests a work item aims at
 constexpr uint32_t kMaxParts = 256;
 constexpr uint32_t kTileGrid = 3072;         // persistent wavefronts
