@@ -64,23 +64,27 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, int lane) {
   return v;
 }
 
-// Pre-pass, one thread per tile: an estimate of the tile's pair tests (its
-// halos times the particles in the rows of the union of their base boxes),
-// the number of parts it is cut into (written to item[t]; item[n] = 0 for
-// the scan that follows), and, when the counts are stored rather than added,
-// zeros in the rows of tiles of several parts (their parts add atomically).
-__global__ void tile_cost_kernel(IndexView ix, HaloView hv, Shells sh, TileView tv,
-                                 uint32_t* __restrict__ item, uint32_t* __restrict__ counts,
-                                 int zero_multi) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// Pre-pass, one wavefront per tile: an estimate of the tile's pair tests
+// (its halos times the particles in the rows of the union of their base
+// boxes; the lanes take the halos, then the rows), the number of parts it
+// is cut into (written to item[t]; item[n] = 0 for the scan that follows),
+// and, when the counts are stored rather than added, zeros in the rows of
+// tiles of several parts (their parts add atomically).
+constexpr int kCostWaves = 4;
+
+__global__ __launch_bounds__(64 * kCostWaves) void tile_cost_kernel(
+    IndexView ix, HaloView hv, Shells sh, TileView tv, uint32_t* __restrict__ item,
+    uint32_t* __restrict__ counts, int zero_multi) {
+  const uint32_t t = blockIdx.x * kCostWaves + (threadIdx.x >> 6);  // same on the wave
+  const int lane = threadIdx.x & 63;
   if (t > tv.n) return;
   if (t == tv.n) {
-    item[t] = 0;
+    if (lane == 0) item[t] = 0;
     return;
   }
   const uint32_t h0 = tv.start[t], nt = tv.len[t];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t i = 0; i < nt; ++i) {
+  for (uint32_t i = lane; i < nt; i += 64) {
     const float c[3] = {hv.x[h0 + i], hv.y[h0 + i], hv.z[h0 + i]};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -90,27 +94,35 @@ __global__ void tile_cost_kernel(IndexView ix, HaloView hv, Shells sh, TileView 
       }
     }
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = wave_min(lo[d]);
+    hi[d] = wave_max(hi[d]);
+  }
   uint64_t cand = 0;
   if (lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2] && ix.n) {
     const int gm1 = ix.grid.g - 1, gxm1 = ix.grid.gx - 1;
     const int x0 = cell_of(lo[0], ix.grid.inv_cell_x, gxm1), x1 = cell_of(hi[0], ix.grid.inv_cell_x, gxm1);
     const int y0 = cell_of(lo[1], ix.grid.inv_cell, gm1), y1 = cell_of(hi[1], ix.grid.inv_cell, gm1);
     const int z0 = cell_of(lo[2], ix.grid.inv_cell, gm1), z1 = cell_of(hi[2], ix.grid.inv_cell, gm1);
-    for (int zz = z0; zz <= z1; ++zz)
-      for (int yy = y0; yy <= y1; ++yy) {
-        const uint32_t kb = ((uint32_t)zz * (uint32_t)ix.grid.g + (uint32_t)yy) * (uint32_t)ix.grid.gx;
-        const uint32_t k0 = kb + (uint32_t)x0, k1 = kb + (uint32_t)x1;
-        cand += ix.cell_end[k1] - (k0 ? ix.cell_end[k0 - 1] : 0u);
-      }
+    const int ny = y1 - y0 + 1, nrows = ny * (z1 - z0 + 1);
+    for (int r = lane; r < nrows; r += 64) {
+      const int zz = z0 + r / ny, yy = y0 + r % ny;
+      const uint32_t kb = ((uint32_t)zz * (uint32_t)ix.grid.g + (uint32_t)yy) * (uint32_t)ix.grid.gx;
+      const uint32_t k0 = kb + (uint32_t)x0, k1 = kb + (uint32_t)x1;
+      cand += ix.cell_end[k1] - (k0 ? ix.cell_end[k0 - 1] : 0u);
+    }
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cand += __shfl_xor(cand, off, 64);
   uint64_t parts = (cand * nt + kItemPairs - 1) / kItemPairs;
   if (parts < 1) parts = 1;
   if (parts > kMaxParts) parts = kMaxParts;
-  item[t] = (uint32_t)parts;
+  if (lane == 0) item[t] = (uint32_t)parts;
   if (zero_multi && parts > 1) {
     for (uint32_t i = 0; i < nt; ++i) {
       uint32_t* dst = counts + (uint64_t)(hv.slot ? hv.slot[h0 + i] : h0 + i) * sh.nrads;
-      for (int k = 0; k < sh.nrads; ++k) dst[k] = 0u;
+      for (int k = lane; k < sh.nrads; k += 64) dst[k] = 0u;
     }
   }
 }
@@ -458,8 +470,8 @@ hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh
   uint32_t* item = work;                       // ntiles + 1
   uint32_t* queue = work + tv.n + 1;           // 1 (own 256-byte line)
   uint32_t* scan_ws = queue + 64;
-  tile_cost_kernel<<<(tv.n + 1 + 255) / 256, 256, 0, s>>>(ix, hv, sh, tv, item, counts,
-                                                          (!measure && !accumulate) ? 1 : 0);
+  tile_cost_kernel<<<(tv.n + 1 + kCostWaves - 1) / kCostWaves, 64 * kCostWaves, 0, s>>>(
+      ix, hv, sh, tv, item, counts, (!measure && !accumulate) ? 1 : 0);
   hipError_t e = exclusive_sum_u32(item, item, (uint64_t)tv.n + 1, scan_ws, s);
   if (e == hipSuccess) e = fill_u32(queue, 0u, 1, s);
   if (e != hipSuccess) return e;
