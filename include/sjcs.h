@@ -173,7 +173,9 @@ int sjcs_sync(sjcs_engine *e);
 int sjcs_get_counts(sjcs_engine *e, uint64_t *counts);
 /* The rows of halos [first, first + nhalos) only, into counts[nhalos][nrads]:
  * a result writer can fetch the next rows while it formats these.  The rows
- * are put back in input order once per state of the totals. */
+ * are put back in input order once per state of the totals.  Both calls copy
+ * more than 64 MiB through the engine's page-locked staging in pieces,
+ * several host threads moving each piece into `counts` (any host memory). */
 int sjcs_get_counts_range(sjcs_engine *e, uint64_t first, uint64_t nhalos, uint64_t *counts);
 
 /* Exact number of in-box (halo image, particle) pair tests the last chunk

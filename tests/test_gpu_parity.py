@@ -1177,6 +1177,10 @@ def test_full_size_config2_properties(eng):
         eng.set_halos(h)
         eng.add_particles_device(d, p.shape[0])
         base = eng.counts()
+        # ranges above the 64 MiB download piece, at an odd offset and at
+        # the end, equal the same rows of the whole result
+        for a, m in ((1_234_567, 300_001), (h.shape[0] - 250_000, 250_000)):
+            assert np.array_equal(eng.counts_range(a, m), base[a:a + m])
         w = eng.last_chunk_pair_tests()
         assert abs(w / 1e9 - 1.0) < 0.005
         print("full size: W", w, "pairs", int(base.sum()), flush=True)
