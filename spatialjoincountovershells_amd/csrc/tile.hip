@@ -135,16 +135,18 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
   extern __shared__ __align__(16) uint32_t smem[];
   const int K = sh.nrads;
   const int kpad = (K + 3) & ~3;
-  // LDS: [r2 kpad][halos 64*kHaloWords][rows 64*3][valid-halo list 64]
-  // [representatives 64][bins kTileBinWords].  The shell table stays in
-  // global memory (L1-resident): a copy per wavefront would cost 8 KB of LDS
-  // and about half of the wavefronts a CU can hold.
-  const ShellEntry* __restrict__ g_lut = sh.lut;
+  // LDS: [r2 kpad][shell table, nlut entries rounded to even][halos
+  // 64*kHaloWords][rows 64*3][valid-halo list 64][representatives 64][bins].
+  // The histograms are sized for the launch's longest tile, so the table's
+  // copy (at most 8 KB) costs few wavefronts per CU and saves each hit an
+  // L1 round trip.
   float* s_r2 = reinterpret_cast<float*>(smem);
   for (int i = threadIdx.x; i < K; i += 64) s_r2[i] = sh.r2[i];
+  ShellEntry* s_lut = reinterpret_cast<ShellEntry*>(smem + kpad);
+  for (int i = threadIdx.x; i < sh.nlut; i += 64) s_lut[i] = sh.lut[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  uint32_t* wbase = smem + kpad;
+  uint32_t* wbase = smem + kpad + 2 * ((sh.nlut + 1) & ~1);
   float* hd = reinterpret_cast<float*>(wbase);
   // per row j of a batch: [64] base (segment q starts at base + q * kTileSeg),
   // [64] first segment, [64] end particle
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(64) void count_tile_kernel(
 #pragma unroll
                 for (int u = 0; u < NB; ++u) {
                   const int b = (int)(__float_as_uint(dist[u]) - sh.lut_base) >> sh.lut_shift;
-                  e[u] = g_lut[hit[u] ? max(b, 0) : 0];
+                  e[u] = s_lut[hit[u] ? max(b, 0) : 0];
                 }
 #pragma unroll
                 for (int u = 0; u < NB; ++u) {
@@ -480,7 +482,8 @@ hipError_t count_tiles(const IndexView& ix, const HaloView& hv, const Shells& sh
   if (tv.max_len < 1 || tv.max_len > (uint32_t)tile_cap(sh.nrads)) return hipErrorInvalidValue;
   const size_t kpad = (size_t)((sh.nrads + 3) & ~3);
   const size_t bin_words = ((size_t)tv.max_len * (size_t)sh.nrads + 3) & ~(size_t)3;
-  const size_t lds = 4 * (kpad + (size_t)kWaveWords - kTileBinWords + bin_words);
+  const size_t lut_words = 2 * (size_t)((sh.nlut + 1) & ~1);
+  const size_t lds = 4 * (kpad + lut_words + (size_t)kWaveWords - kTileBinWords + bin_words);
   const int acc = accumulate ? 1 : 0;
   if (sh.one_step) {
     if (measure)
