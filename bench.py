@@ -310,15 +310,23 @@ def serve_bench(args, cfg):
         tk = sv.submit(h, r2)
         sv.wait(tk)
         sv.release(tk)
-    log("[bench] capacity run")
-    t0 = time.perf_counter()
-    tickets = [sv.submit(h, r2) for h, r2 in reqs]
-    for tk in tickets:
-        sv.wait(tk)
-    cap_s = time.perf_counter() - t0
+    # capacity: every request submitted at once, twice.  The first pass's
+    # output arrays are fresh host memory, each page faulted in as the
+    # counts land (glibc maps large blocks anew); the second pass reuses
+    # what the first returned, as a caller recycling its result buffers
+    # does, and is the figure reported.
+    cap_runs = []
+    for rep in range(2):
+        log(f"[bench] capacity run {rep + 1}")
+        t0 = time.perf_counter()
+        tickets = [sv.submit(h, r2) for h, r2 in reqs]
+        for tk in tickets:
+            sv.wait(tk)
+        cap_runs.append(time.perf_counter() - t0)
+        for tk in tickets:
+            sv.release(tk)
+    cap_s = cap_runs[-1]
     capacity = len(reqs) / cap_s
-    for tk in tickets:
-        sv.release(tk)
     rate = args.load * capacity
     log(f"[bench] capacity {capacity:.1f} req/s; Poisson arrivals at {rate:.1f} req/s")
     gaps = rng.exponential(1.0 / rate, len(reqs))
@@ -357,7 +365,9 @@ def serve_bench(args, cfg):
         "config": {"workload": cfg["workload"], "particles": cfg["np"], "requests": len(reqs),
                    "halos": halos, "slots": 8, "parallelism": "single GPU"},
         "capacity": {"requests_per_s": capacity, "halos_per_s": halos / cap_s,
-                     "wall_s": round(cap_s, 3)},
+                     "wall_s": round(cap_s, 3),
+                     "first_pass_requests_per_s": len(reqs) / cap_runs[0],
+                     "note": "second of two passes; the first writes into fresh host memory"},
         "poisson": {"rate_per_s": rate, "load": args.load, "wall_s": round(run_s, 3),
                     "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                    "p99": float(np.percentile(lat, 99)),

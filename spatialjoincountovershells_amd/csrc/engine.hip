@@ -2209,11 +2209,17 @@ int serve_launch(sjcs_server* sv, ServerSlot& sl, ServerRequest& r) {
   const int lanes_auto = !((double)ix.n * side * side * side <= 4096.0) ? 64 : 16;
   SRVCALL(count_halos(ix, hv, r.shells, sl.c32.as<uint32_t>(), false, nullptr, lanes_auto, 4, sl.st),
           "count");
-  // the other key buffer is free after the sort: it takes the inverse rows
-  uint32_t* inv = sl.key[sel ^ 1].as<uint32_t>();
-  SRVCALL(invert_rows(sl.row[sel].as<uint32_t>(), inv, n, sl.st), "read out");
-  SRVCALL(readout_counts(sl.c32.as<uint32_t>(), nullptr, inv, sl.c64.as<uint64_t>(), n, K, sl.st),
-          "read out");
+  if (n >= kServeSortMin) {
+    // the other key buffer is free after the sort: it takes the inverse rows
+    uint32_t* inv = sl.key[sel ^ 1].as<uint32_t>();
+    SRVCALL(invert_rows(sl.row[sel].as<uint32_t>(), inv, n, sl.st), "read out");
+    SRVCALL(readout_counts(sl.c32.as<uint32_t>(), nullptr, inv, sl.c64.as<uint64_t>(), n, K, sl.st),
+            "read out");
+  } else {
+    // unsorted: the count rows are the input rows, widened in place order
+    SRVCALL(widen_counts(sl.c32.as<uint32_t>(), sl.c64.as<uint64_t>(), nc, true, sl.st),
+            "read out");
+  }
   if (nc)
     SRVCALL(hipMemcpyAsync(sl.pin, sl.c64.p, 8 * nc, hipMemcpyDeviceToHost, sl.st), "hipMemcpy");
   SRVCALL(hipEventRecord(sl.done, sl.st), "hipEventRecord");

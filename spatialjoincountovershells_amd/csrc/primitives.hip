@@ -147,9 +147,34 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(
   }
 }
 
+// A short array scanned by one block in chunks of 1024 (in place allowed:
+// a chunk's reads precede its writes across the scan's barrier).
+template <class Op, bool kExclusive>
+__global__ __launch_bounds__(1024) void scan_single_kernel(const uint32_t* in, uint32_t* out,
+                                                           uint32_t n) {
+  __shared__ uint32_t s_tmp[16];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t exc = block_excl_scan<Op, 1024>(v, s_tmp, &tot);
+    if (i < n) out[i] = Op::apply(carry, kExclusive ? exc : Op::apply(exc, v));
+    carry = Op::apply(carry, tot);
+  }
+}
+
+// Arrays up to this length take the one-launch scan (a serving request's
+// radix histograms, small tile lists): one launch instead of three.
+constexpr uint64_t kScanSingleMax = 8192;
+
 template <class Op, bool kExclusive>
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* ws, hipStream_t s) {
   if (n == 0) return hipSuccess;
+  if (n <= kScanSingleMax) {
+    scan_single_kernel<Op, kExclusive><<<1, 1024, 0, s>>>(in, out, (uint32_t)n);
+    return hipGetLastError();
+  }
   uint64_t tiles = ceil_div(n, kScanTile);
   if (tiles > 0xffffffffull) return hipErrorInvalidValue;
   scan_reduce_kernel<Op><<<(uint32_t)tiles, kScanThreads, 0, s>>>(in, n, ws);
