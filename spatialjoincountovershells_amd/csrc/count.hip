@@ -450,12 +450,13 @@ __global__ __launch_bounds__(kCountThreads) void count_group_kernel(
         const uint32_t aend = beg + len;
         const uint32_t max_stotal = wave_max_of_groups<GROUP>(stotal);
         uint32_t carry = 0;
-        // 64 lanes per halo: the next round's candidates are loaded before
-        // this round's are tested (its loads wait out the tests).  Fewer
-        // lanes per halo keep one round in flight: the second set of
-        // registers costs them more occupancy than the overlap gains.
-        constexpr bool kAhead = GROUP == 64;
-        // Round c0's candidates of this lane (64 lanes: never staged), as in
+        // 64 and 16 lanes per halo (without LDS staging): the next round's
+        // candidates are loaded before this round's are tested (its loads
+        // wait out the tests; at 16 lanes the second set of registers leaves
+        // 5 wavefronts per SIMD instead of 6, and still gains).  8 lanes keep
+        // one round in flight.
+        constexpr bool kAhead = GROUP == 64 || (GROUP == 16 && !kStage);
+        // Round c0's candidates of this lane (never staged), as in
         // the round code below, for the round ahead.  Called by the whole
         // wave, rounds in order (carry).
         auto fetch_ahead = [&](uint32_t c0, float (&fx)[SEG], float (&fy)[SEG],
