@@ -1168,8 +1168,12 @@ int sjcs_engine_create(int device, sjcs_engine** out) {
     if (h == hipSuccess) h = hipEventRecord(e->uploaded[i], e->stream);
     if (h == hipSuccess) h = hipEventRecord(e->stage_read[i], e->stream);
   }
-  // the pair-test total (u64) and the halo sort's run flag (u32)
+  // the pair-test total (u64) and the halo sort's run flag (u32); clearing
+  // it is the process's first kernel of this library, which loads the
+  // library's code onto the device here rather than in the first halo sort
   if (h == hipSuccess) h = ensure(e->inbox, 16);
+  if (h == hipSuccess) h = fill_u32(e->inbox.as<uint32_t>(), 0u, 4, e->stream);
+  if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
   if (h != hipSuccess) {
     g_create_error = std::string("engine setup: ") + hipGetErrorString(h);
     sjcs_engine_destroy(e);
