@@ -720,7 +720,9 @@ __global__ void invert_rows_kernel(const uint32_t* __restrict__ orig, uint32_t* 
 // may be absent): the rows back in input order.  A block writes
 // kReadoutRows consecutive output rows, its lanes walking their
 // kReadoutRows * K words in order, so the u64 stores are contiguous and each
-// source row is one contiguous read of K words.
+// source row is one contiguous read of K words.  Sources and output are
+// touched once, so both go non-temporal (no L2 retention): 1.40 -> 1.33 ms
+// for 1e7 x 40 counts.
 constexpr uint32_t kReadoutRows = 64;
 constexpr int kReadoutUnroll = 4;
 
@@ -744,7 +746,8 @@ __global__ __launch_bounds__(256) void readout_counts_kernel(
         v[u] = 0;
         if (o0 + 256u * u < m) {
           const uint64_t src = (uint64_t)inv[r0 + r] * K + k;
-          v[u] = (c64 ? c64[src] : 0ull) + (c32 ? c32[src] : 0u);
+          v[u] = (c64 ? __builtin_nontemporal_load(&c64[src]) : 0ull) +
+                 (c32 ? __builtin_nontemporal_load(&c32[src]) : 0u);
         }
         r += step_r;
         k += step_k;
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(256) void readout_counts_kernel(
       }
 #pragma unroll
       for (int u = 0; u < kReadoutUnroll; ++u)
-        if (o0 + 256u * u < m) d[o0 + 256u * u] = v[u];
+        if (o0 + 256u * u < m) __builtin_nontemporal_store(v[u], &d[o0 + 256u * u]);
     }
   }
 }
