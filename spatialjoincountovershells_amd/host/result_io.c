@@ -88,6 +88,8 @@ typedef struct {
     const uint64_t *rows;   /* rows of the batch, from halo `first` */
     uint64_t first;
     const label_t *labels;
+    const char *slots;      /* every label in a LABEL_SLOT-byte slot, or NULL */
+    const unsigned char *slot_len;
     int nrads;
     uint64_t beg, end;
     uint64_t *tot;          /* this worker's share of the totals */
@@ -136,6 +138,48 @@ static void *pool_run(void *arg)
     return NULL;
 }
 
+/* The rows of job j when every label fits a slot: the labels are read from
+ * one table (no pointer per line, nothing reloaded after each store through
+ * the output pointer), counts below 100 take one path whichever their
+ * digit count, and the totals are summed in a second pass over the rows. */
+static void fmt_rows_slots(fmt_job *j)
+{
+    char *restrict p = j->buf;
+    const int K = j->nrads;
+    const char *restrict slots = j->slots;
+    const unsigned char *restrict slen = j->slot_len;
+    const sjcs_halo *restrict halo = j->h->halo;
+    const uint64_t *restrict rows = j->rows + (j->beg - j->first) * (uint64_t)K;
+    uint64_t *restrict tot = j->tot;
+    uint64_t n;
+    int k;
+    for (n = j->beg; n < j->end; n++, rows += K) {
+        memcpy(p, "id ", 3);
+        p = put_u64(p + 3, halo[n].id);
+        *p++ = '\n';
+        for (k = 0; k < K; k++) {
+            const uint64_t v = rows[k];
+            __builtin_memcpy(p, slots + (size_t)k * LABEL_SLOT, LABEL_SLOT);
+            p += slen[k];
+            if (v < 100) {
+                const unsigned two = v >= 10;
+                p[0] = digits2[2 * v + 1 - two];
+                p[1] = digits2[2 * v + 1];
+                p[1 + two] = '\n';
+                p += 2 + two;
+            } else {
+                p = put_u64(p, v);
+                *p++ = '\n';
+            }
+        }
+    }
+    j->len = (size_t)(p - j->buf);
+    rows = j->rows + (j->beg - j->first) * (uint64_t)K;
+    for (n = j->beg; n < j->end; n++, rows += K)
+        for (k = 0; k < K; k++)
+            tot[k] += rows[k];
+}
+
 static void fmt_batch(fmt_job *j)
 {
     size_t maxline = 0, need;
@@ -153,6 +197,10 @@ static void fmt_batch(fmt_job *j)
     }
     if (j->buf == NULL) {
         j->len = (size_t)-1;
+        return;
+    }
+    if (j->slots) {
+        fmt_rows_slots(j);
         return;
     }
     p = j->buf;
@@ -198,6 +246,8 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
                            const float *radiuses_sq, int nrads, int nthreads)
 {
     label_t *labels;
+    char *slots;
+    unsigned char *slot_len;
     uint64_t *total, *row0;
     const uint64_t *cur, *next;
     fmt_job *jobs;
@@ -243,6 +293,21 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
         }
     }
 
+    slots = malloc((size_t)nrads * LABEL_SLOT + 1);
+    slot_len = malloc((size_t)nrads + 1);
+    for (k = 0; k < nrads && slots && slot_len; k++) {
+        if (labels[k].len > LABEL_SLOT)
+            break;
+        memcpy(slots + (size_t)k * LABEL_SLOT, labels[k].s, LABEL_SLOT);
+        slot_len[k] = (unsigned char)labels[k].len;
+    }
+    if (k < nrads) {
+        free(slots);
+        free(slot_len);
+        slots = NULL;
+        slot_len = NULL;
+    }
+
     if (fname == NULL) {
         fp = stdout;
     } else {
@@ -252,6 +317,8 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
             for (k = 0; k < nrads; k++)
                 free(labels[k].s);
             free(labels);
+            free(slots);
+            free(slot_len);
             return -1;
         }
     }
@@ -296,6 +363,8 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
             jobs[t].rows = cur;
             jobs[t].first = b;
             jobs[t].labels = labels;
+            jobs[t].slots = slots;
+            jobs[t].slot_len = slot_len;
             jobs[t].nrads = nrads;
             jobs[t].beg = b + (e - b) * (uint64_t)t / (uint64_t)nthreads;
             jobs[t].end = b + (e - b) * (uint64_t)(t + 1) / (uint64_t)nthreads;
@@ -378,6 +447,8 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
     for (k = 0; k < nrads; k++)
         free(labels[k].s);
     free(labels);
+    free(slots);
+    free(slot_len);
     return rc;
 }
 

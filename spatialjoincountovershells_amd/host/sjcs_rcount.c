@@ -101,13 +101,16 @@ static void *setup_run(void *arg)
 {
     setup_job *s = arg;
     s->rc = sjcs_engine_create(s->device, &s->e);
+    trace("  (set-up thread) engine created");
     if (s->rc == SJCS_OK)
         s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[0]);
     if (s->rc == SJCS_OK)
         s->rc = sjcs_host_alloc(PIECE_BYTES, &s->pin[1]);
+    trace("  (set-up thread) page-locked pieces");
     if (s->rc == SJCS_OK && s->first)
         s->read_rc = read_chunk(s->e, s->pin, s->first, s->buf, s->compressed, s->nthreads,
                                 s->err, sizeof(s->err));
+    trace("  (set-up thread) first particle file on the device");
     return NULL;
 }
 
@@ -168,6 +171,85 @@ static int file_to_device(sjcs_engine *e, void *pin[2], const char *fname, uint6
         else if (sjcs_copy_to_device(e, d + done, pin[k], (size_t)len) != SJCS_OK)
             return -3;
         done += len;
+        k ^= 1;
+    }
+    if (copying) {
+        pthread_join(th, NULL);
+        if (job.rc != SJCS_OK)
+            return -3;
+    }
+    return 0;
+}
+
+/* Halo centres [beg, end) packed as FP32 xyz at dst, by several threads. */
+typedef struct {
+    const sjcs_halo *h;
+    uint64_t beg, end;
+    float *dst;
+} centre_pack;
+
+static void *centre_pack_run(void *arg)
+{
+    const centre_pack *c = arg;
+    uint64_t i;
+    for (i = c->beg; i < c->end; i++)
+        memcpy(c->dst + 3 * (i - c->beg), c->h[i].point, 12);
+    return NULL;
+}
+
+static void pack_centres_par(const sjcs_halo *h, uint64_t n, float *dst, int nthreads)
+{
+    pthread_t th[64];
+    centre_pack job[64];
+    int t, nt = nthreads < 1 ? 1 : (nthreads > 64 ? 64 : nthreads), started = 0;
+    for (t = 0; t < nt; t++) {
+        job[t].h = h;
+        job[t].beg = n * (uint64_t)t / (uint64_t)nt;
+        job[t].end = n * (uint64_t)(t + 1) / (uint64_t)nt;
+        job[t].dst = dst + 3 * job[t].beg;
+    }
+    for (t = 1; t < nt; t++) {
+        if (pthread_create(&th[t], NULL, centre_pack_run, &job[t]) != 0)
+            break;
+        started = t;
+    }
+    centre_pack_run(&job[0]);
+    for (t = started + 1; t < nt; t++)  /* threads that could not be started */
+        centre_pack_run(&job[t]);
+    for (t = 1; t <= started; t++)
+        pthread_join(th[t], NULL);
+}
+
+/* The halo centres into device memory at d (12 bytes each) through the two
+ * pinned pieces: piece i is copied to the device on a helper thread while
+ * piece i + 1 is packed into the other one.  0, or -3 (copy failed). */
+static int centres_to_device(sjcs_engine *e, void *pin[2], const sjcs_halos *halos, char *d,
+                             int nthreads)
+{
+    const uint64_t per = PIECE_BYTES / 12;
+    uint64_t first = 0;
+    int k = 0, copying = 0;
+    pthread_t th;
+    piece_copy job;
+    while (first < halos->n) {
+        const uint64_t cnt = halos->n - first < per ? halos->n - first : per;
+        pack_centres_par(halos->halo + first, cnt, pin[k], nthreads);
+        if (copying) {  /* the other piece's copy; pin[k ^ 1] is free after it */
+            pthread_join(th, NULL);
+            copying = 0;
+            if (job.rc != SJCS_OK)
+                return -3;
+        }
+        job.e = e;
+        job.dst = d + 12 * first;
+        job.src = pin[k];
+        job.len = (size_t)(12 * cnt);
+        job.rc = SJCS_OK;
+        if (pthread_create(&th, NULL, piece_copy_run, &job) == 0)
+            copying = 1;
+        else if (sjcs_copy_to_device(e, job.dst, pin[k], job.len) != SJCS_OK)
+            return -3;
+        first += cnt;
         k ^= 1;
     }
     if (copying) {
@@ -472,10 +554,12 @@ int main(int argc, char *argv[])
     if (!setup_threaded)
         setup_run(&setup);
 
-    /* halos (main.c:634-642) */
+    /* halos (main.c:634-642); one thread fewer than the host threads, so
+     * the device set-up (runtime start, code load) is not starved meanwhile */
     TIMER_START(tv);
     if (cli.filter_file == NULL)
-        rc = sjcs_read_halos(cli.halo_file, &halos, hthreads, err, sizeof(err));
+        rc = sjcs_read_halos(cli.halo_file, &halos, hthreads > 1 ? hthreads - 1 : 1, err,
+                             sizeof(err));
     else
         rc = sjcs_read_and_filter_halos(cli.halo_file, cli.filter_file, &halos, err, sizeof(err));
     if (rc != 0) {
@@ -534,9 +618,20 @@ int main(int argc, char *argv[])
     }
     if (sjcs_set_shells(e, r2, cli.nrads) != SJCS_OK)
         die_engine(e, "shells");
-    if (sjcs_set_halos(e, halos.n ? halos.halo[0].point : NULL, sizeof(sjcs_halo), halos.n) !=
-        SJCS_OK)
-        die_engine(e, "halos");
+    trace("shells on the device");
+    /* the centres go up packed (12 of the records' 24 bytes) through the
+     * page-locked pieces: a copy of the records from pageable memory runs
+     * at a fraction of the link's rate */
+    {
+        void *d_centres = NULL;
+        if (halos.n && (sjcs_device_alloc(e, 12 * (size_t)halos.n, &d_centres) != SJCS_OK ||
+                        centres_to_device(e, setup.pin, &halos, d_centres, hthreads) != 0))
+            die_engine(e, "halos");
+        if (sjcs_set_halos_device(e, d_centres, halos.n) != SJCS_OK)
+            die_engine(e, "halos");
+        if (d_centres)
+            sjcs_device_free(e, d_centres);
+    }
     trace("shells and halos on the device");
 
     /* chunk loop (main.c:235-368) */

@@ -180,20 +180,25 @@ def test_plain_formatter_equals_reference_files(case):
     assert t.format_result(ids, tot, r2) == t.read_expected_text(case["expected"])
 
 
-@pytest.mark.parametrize("spec,nthreads", [("0.001:5:40", 1), ("0.001:5:40", 16),
-                                           ("0.01:20:64", 16), ("2:3:1", 7)])
-def test_result_writer_many_batches(host, tmp_path, spec, nthreads):
+@pytest.mark.parametrize("spec,nthreads,top", [("0.001:5:40", 1, 1 << 20),
+                                               ("0.001:5:40", 16, 1 << 20),
+                                               ("0.01:20:64", 16, 1 << 20), ("2:3:1", 7, 1 << 20),
+                                               ("0.001:5:40", 16, 150),
+                                               ("100000:900000:3", 4, 150)])
+def test_result_writer_many_batches(host, tmp_path, spec, nthreads, top):
     """140000 halos -- five of the writer's 32768-row batches, the last one
     partial -- with ids and counts over the whole u64 range (sums that wrap
-    in the Total block included), written by the host writer with 1 to 16
-    threads: byte for byte the plain formatter's text."""
-    rng = np.random.default_rng(len(spec) * 31 + nthreads)
+    in the Total block included), or counts mostly of one and two digits
+    (`top` 150), with labels of at most 32 bytes (the slot table) or longer
+    (radii of six digits), written by the host writer with 1 to 16 threads:
+    byte for byte the plain formatter's text."""
+    rng = np.random.default_rng(len(spec) * 31 + nthreads + top)
     mn, mx, k = t.parse_radius(spec)
     _, r2 = t.oracle_thresholds(mn, mx, k)
     n = 140_000
     ids = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
     ids[:3] = [0, 1, np.iinfo(np.uint64).max]
-    counts = rng.integers(0, 1 << 20, (n, k), dtype=np.uint64)
+    counts = rng.integers(0, top, (n, k), dtype=np.uint64)
     counts[rng.random((n, k)) < 0.001] = np.iinfo(np.uint64).max - np.uint64(5)
     counts[0, 0] = np.iinfo(np.uint64).max  # halo 0 is doubled in the Total block
     arr = (Halo * n)()
