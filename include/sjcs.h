@@ -177,6 +177,14 @@ int sjcs_get_counts(sjcs_engine *e, uint64_t *counts);
  * more than 64 MiB through the engine's page-locked staging in pieces,
  * several host threads moving each piece into `counts` (any host memory). */
 int sjcs_get_counts_range(sjcs_engine *e, uint64_t first, uint64_t nhalos, uint64_t *counts);
+/* The same rows as uint32, half the bytes over the link, while every total
+ * is still one of the device's 32-bit running counts -- no fold into 64
+ * bits has happened since the last reset (sjcs_reset_counts, sjcs_set_*):
+ * the values are then exact.  Otherwise SJCS_ERR_STATE (take
+ * sjcs_get_counts_range); nhalos = 0 asks which case holds.  No reference
+ * counterpart: the read-back of a result writer that formats 32-bit rows. */
+int sjcs_get_counts_range_u32(sjcs_engine *e, uint64_t first, uint64_t nhalos,
+                              uint32_t *counts);
 
 /* Exact number of in-box (halo image, particle) pair tests the last chunk
  * made -- the reference's CalcDistance calls (MultiRangeCount.c:107).  Runs

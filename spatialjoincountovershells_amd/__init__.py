@@ -49,7 +49,7 @@ EXPORTED = (
     "sjcs_index_create", "sjcs_index_insert", "sjcs_index_build",
     "sjcs_index_count", "sjcs_index_destroy", "sjcs_index_count_noalloc",
     "sjcs_device_alloc", "sjcs_device_free", "sjcs_copy_to_device", "sjcs_copy_to_host",
-    "sjcs_host_alloc", "sjcs_host_free", "sjcs_get_counts_range",
+    "sjcs_host_alloc", "sjcs_host_free", "sjcs_get_counts_range", "sjcs_get_counts_range_u32",
     "sjcs_gadget_decode_params", "sjcs_add_compressed_particles",
     "sjcs_add_compressed_particles_device",
     "sjcs_decode_compressed_device",
@@ -200,6 +200,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "sjcs_host_alloc": (i32, [sz, pp]),
         "sjcs_host_free": (None, [vp]),
         "sjcs_get_counts_range": (i32, [vp, u64, u64, vp]),
+        "sjcs_get_counts_range_u32": (i32, [vp, u64, u64, vp]),
         "sjcs_gadget_decode_params": (i32, [vp, i32, ctypes.POINTER(GadgetDecode)]),
         "sjcs_add_compressed_particles": (i32, [vp, vp, u64, ctypes.POINTER(GadgetDecode)]),
         "sjcs_add_compressed_particles_device": (i32, [vp, vp, u64,
@@ -339,6 +340,17 @@ class Engine:
         out = np.zeros((n, self.nrads), dtype=np.uint64)
         self._check(self._lib.sjcs_get_counts_range(self._h, first, n, _ptr(out)))
         return out
+
+    def counts_range_u32(self, first: int, n: int) -> np.ndarray:
+        """Rows [first, first + n) of the counts as uint32; SJCSError
+        (SJCS_ERR_STATE) once the totals are held in 64 bits."""
+        out = np.zeros((n, self.nrads), dtype=np.uint32)
+        self._check(self._lib.sjcs_get_counts_range_u32(self._h, first, n, _ptr(out)))
+        return out
+
+    def counts_fit_u32(self) -> bool:
+        """Whether counts_range_u32 can serve the current totals."""
+        return self._lib.sjcs_get_counts_range_u32(self._h, 0, 0, None) == SJCS_OK
 
     def last_chunk_pair_tests(self) -> int:
         v = ctypes.c_uint64()

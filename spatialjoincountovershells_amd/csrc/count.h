@@ -56,5 +56,8 @@ hipError_t invert_rows(const uint32_t* orig, uint32_t* inv, uint64_t n, hipStrea
 // being input row r's sorted position (invert_rows).  Requires K*64 < 2^32.
 hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32_t* inv,
                           uint64_t* dst, uint64_t n, int K, hipStream_t s);
+// The same rows from the u32 counts alone, as u32.
+hipError_t readout_counts32(const uint32_t* c32, const uint32_t* inv, uint32_t* dst, uint64_t n,
+                            int K, hipStream_t s);
 
 }  // namespace sjcs

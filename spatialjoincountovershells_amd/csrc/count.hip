@@ -717,18 +717,19 @@ __global__ void invert_rows_kernel(const uint32_t* __restrict__ orig, uint32_t* 
 }
 
 // dst[r * K + k] = c64[inv[r] * K + k] + c32[inv[r] * K + k] (either source
-// may be absent): the rows back in input order.  A block writes
-// kReadoutRows consecutive output rows, its lanes walking their
-// kReadoutRows * K words in order, so the u64 stores are contiguous and each
-// source row is one contiguous read of K words.  Sources and output are
-// touched once, so both go non-temporal (no L2 retention): 1.40 -> 1.33 ms
-// for 1e7 x 40 counts.
+// may be absent; a u32 destination takes c32 alone): the rows back in input
+// order.  A block writes kReadoutRows consecutive output rows, its lanes
+// walking their kReadoutRows * K words in order, so the stores are
+// contiguous and each source row is one contiguous read of K words.  Sources
+// and output are touched once, so both go non-temporal (no L2 retention):
+// 1.40 -> 1.33 ms for 1e7 x 40 u64 counts.
 constexpr uint32_t kReadoutRows = 64;
 constexpr int kReadoutUnroll = 4;
 
+template <typename Out>
 __global__ __launch_bounds__(256) void readout_counts_kernel(
     const uint32_t* __restrict__ c32, const uint64_t* __restrict__ c64,
-    const uint32_t* __restrict__ inv, uint64_t* __restrict__ dst, uint64_t n, uint32_t K) {
+    const uint32_t* __restrict__ inv, Out* __restrict__ dst, uint64_t n, uint32_t K) {
   // a thread's word o = row r * K + k advances by 256 words a step: r and k
   // are carried along instead of dividing every word's index by K
   const uint32_t step_r = 256u / K, step_k = 256u % K;
@@ -737,17 +738,20 @@ __global__ __launch_bounds__(256) void readout_counts_kernel(
        r0 += (uint64_t)gridDim.x * kReadoutRows) {
     const uint32_t rows = (uint32_t)min((uint64_t)kReadoutRows, n - r0);
     const uint32_t m = rows * K;
-    uint64_t* d = dst + r0 * K;
+    Out* d = dst + r0 * K;
     uint32_t r = r_first, k = k_first;
     for (uint32_t o0 = threadIdx.x; o0 < m; o0 += 256 * kReadoutUnroll) {
-      uint64_t v[kReadoutUnroll];
+      Out v[kReadoutUnroll];
 #pragma unroll
       for (int u = 0; u < kReadoutUnroll; ++u) {
         v[u] = 0;
         if (o0 + 256u * u < m) {
           const uint64_t src = (uint64_t)inv[r0 + r] * K + k;
-          v[u] = (c64 ? __builtin_nontemporal_load(&c64[src]) : 0ull) +
-                 (c32 ? __builtin_nontemporal_load(&c32[src]) : 0u);
+          if constexpr (sizeof(Out) == 8)
+            v[u] = (c64 ? __builtin_nontemporal_load(&c64[src]) : 0ull) +
+                   (c32 ? __builtin_nontemporal_load(&c32[src]) : 0u);
+          else
+            v[u] = c32 ? __builtin_nontemporal_load(&c32[src]) : 0u;
         }
         r += step_r;
         k += step_k;
@@ -787,7 +791,18 @@ hipError_t readout_counts(const uint32_t* c32, const uint64_t* c64, const uint32
   if (n == 0 || K <= 0) return hipSuccess;
   uint64_t blocks = ceil_div(n, kReadoutRows);
   if (blocks > 16384) blocks = 16384;
-  readout_counts_kernel<<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, inv, dst, n, (uint32_t)K);
+  readout_counts_kernel<uint64_t><<<(uint32_t)blocks, 256, 0, s>>>(c32, c64, inv, dst, n,
+                                                                   (uint32_t)K);
+  return hipGetLastError();
+}
+
+hipError_t readout_counts32(const uint32_t* c32, const uint32_t* inv, uint32_t* dst, uint64_t n,
+                            int K, hipStream_t s) {
+  if (n == 0 || K <= 0) return hipSuccess;
+  uint64_t blocks = ceil_div(n, kReadoutRows);
+  if (blocks > 16384) blocks = 16384;
+  readout_counts_kernel<uint32_t><<<(uint32_t)blocks, 256, 0, s>>>(c32, nullptr, inv, dst, n,
+                                                                   (uint32_t)K);
   return hipGetLastError();
 }
 

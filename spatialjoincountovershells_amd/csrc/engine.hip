@@ -258,12 +258,13 @@ struct sjcs_engine {
   // count could reach 2^32 (a particle counts at most once per image).
   DevBuf counts;    // u32 [nhalos][K]
   DevBuf counts64;  // u64 [nhalos][K]
-  DevBuf ucounts;   // u64, input order, for read-out
+  DevBuf ucounts;   // input order, for read-out: u64, or u32 (readout_u32)
   bool counts_fresh = true;  // the u32 counts hold nothing: next chunk stores
   bool c64_live = false;     // counts64 holds totals
   uint64_t pending = 0;      // bound on any u32 count
   uint64_t counts_gen = 1;   // changes whenever the totals may change
   uint64_t readout_gen = 0;  // totals `ucounts` holds (input row order)
+  bool readout_u32 = false;  // ... as u32 (only while !c64_live)
 
 
   // Chunk i uses slot i % 2: its upload (bstream) overlaps the count of
@@ -1366,37 +1367,59 @@ int sjcs_get_counts(sjcs_engine* e, uint64_t* counts) {
   return sjcs_get_counts_range(e, 0, e->halos.n, counts);
 }
 
-int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint64_t* counts) {
+// Rows [first, first + nhalos) of the totals in input order, u64 or (while
+// the totals are the u32 running counts alone) u32.
+int get_counts_rows(sjcs_engine* e, uint64_t first, uint64_t nhalos, void* counts, bool u32) {
   int rc = check_ready(e);
   if (rc != SJCS_OK) return rc;
   if (first > e->halos.n || nhalos > e->halos.n - first)
     return fail(e, SJCS_ERR_INVALID, "counts: halo range out of bounds");
   const size_t K = (size_t)e->shells.nrads;
   const size_t n = (size_t)nhalos * K;
+  const size_t w = u32 ? sizeof(uint32_t) : sizeof(uint64_t);
   if (!counts && n) return fail(e, SJCS_ERR_INVALID, "counts: null pointer");
   rc = quiesce(e);
   if (rc != SJCS_OK) return rc;
+  if (u32 && e->c64_live)
+    return fail(e, SJCS_ERR_STATE,
+                "counts: the totals are held in 64 bits; use sjcs_get_counts_range");
   if (n == 0) return SJCS_OK;
   if (e->counts_fresh && !e->c64_live) {
-    memset(counts, 0, n * sizeof(uint64_t));
+    memset(counts, 0, n * w);
     return SJCS_OK;
   }
-  // every row back in input order once per state of the totals; ranges are
-  // then plain copies
-  if (e->readout_gen != e->counts_gen) {
+  // every row back in input order once per state of the totals (and width);
+  // ranges are then plain copies
+  if (e->readout_gen != e->counts_gen || e->readout_u32 != u32) {
     const size_t all = (size_t)e->halos.n * K;
-    HIPCALL(e, ensure(e->ucounts, all * sizeof(uint64_t)), "hipMalloc(counts)");
+    HIPCALL(e, ensure(e->ucounts, all * w), "hipMalloc(counts)");
     HIPCALL(e, hipEventRecord(e->rev[0], e->stream), "hipEventRecord");
-    HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
-                              e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
-                              e->halos.inv.as<uint32_t>(), e->ucounts.as<uint64_t>(), e->halos.n,
-                              e->shells.nrads, e->stream),
-            "read out counts");
+    if (u32)
+      HIPCALL(e, readout_counts32(e->counts.as<uint32_t>(), e->halos.inv.as<uint32_t>(),
+                                  e->ucounts.as<uint32_t>(), e->halos.n, e->shells.nrads,
+                                  e->stream),
+              "read out counts");
+    else
+      HIPCALL(e, readout_counts(e->counts_fresh ? nullptr : e->counts.as<uint32_t>(),
+                                e->c64_live ? e->counts64.as<uint64_t>() : nullptr,
+                                e->halos.inv.as<uint32_t>(), e->ucounts.as<uint64_t>(),
+                                e->halos.n, e->shells.nrads, e->stream),
+              "read out counts");
     HIPCALL(e, hipEventRecord(e->rev[1], e->stream), "hipEventRecord");
     e->readout_timed = true;
     e->readout_gen = e->counts_gen;
+    e->readout_u32 = u32;
   }
-  return download(e, counts, e->ucounts.as<uint64_t>() + first * K, n * sizeof(uint64_t));
+  return download(e, counts, e->ucounts.as<char>() + first * K * w, n * w);
+}
+
+int sjcs_get_counts_range(sjcs_engine* e, uint64_t first, uint64_t nhalos, uint64_t* counts) {
+  return get_counts_rows(e, first, nhalos, counts, false);
+}
+
+int sjcs_get_counts_range_u32(sjcs_engine* e, uint64_t first, uint64_t nhalos,
+                              uint32_t* counts) {
+  return get_counts_rows(e, first, nhalos, counts, true);
 }
 
 int sjcs_last_chunk_pair_tests(sjcs_engine* e, uint64_t* pair_tests) {

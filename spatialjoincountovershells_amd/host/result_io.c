@@ -85,7 +85,8 @@ static inline char *put_u64(char *p, uint64_t v)
 
 typedef struct {
     const sjcs_halos *h;
-    const uint64_t *rows;   /* rows of the batch, from halo `first` */
+    const void *rows;       /* rows of the batch, from halo `first` */
+    int wide;               /* u64 rows (else u32) */
     uint64_t first;
     const label_t *labels;
     const char *slots;      /* every label in a LABEL_SLOT-byte slot, or NULL */
@@ -138,27 +139,32 @@ static void *pool_run(void *arg)
     return NULL;
 }
 
+/* Count i of a block of u64 (wide) or u32 rows. */
+#define ROW_AT(rows, wide, i) \
+    ((wide) ? ((const uint64_t *)(rows))[i] : (uint64_t)((const uint32_t *)(rows))[i])
+
 /* The rows of job j when every label fits a slot: the labels are read from
  * one table (no pointer per line, nothing reloaded after each store through
  * the output pointer), counts below 100 take one path whichever their
- * digit count, and the totals are summed in a second pass over the rows. */
-static void fmt_rows_slots(fmt_job *j)
+ * digit count, and the totals are summed in a second pass over the rows.
+ * Inlined with `wide` a constant, one copy per row width. */
+static inline __attribute__((always_inline)) void fmt_rows_slots_w(fmt_job *j, const int wide)
 {
     char *restrict p = j->buf;
     const int K = j->nrads;
     const char *restrict slots = j->slots;
     const unsigned char *restrict slen = j->slot_len;
     const sjcs_halo *restrict halo = j->h->halo;
-    const uint64_t *restrict rows = j->rows + (j->beg - j->first) * (uint64_t)K;
+    const void *rows = j->rows;
     uint64_t *restrict tot = j->tot;
-    uint64_t n;
+    uint64_t n, i = (j->beg - j->first) * (uint64_t)K;
     int k;
-    for (n = j->beg; n < j->end; n++, rows += K) {
+    for (n = j->beg; n < j->end; n++) {
         memcpy(p, "id ", 3);
         p = put_u64(p + 3, halo[n].id);
         *p++ = '\n';
-        for (k = 0; k < K; k++) {
-            const uint64_t v = rows[k];
+        for (k = 0; k < K; k++, i++) {
+            const uint64_t v = ROW_AT(rows, wide, i);
             __builtin_memcpy(p, slots + (size_t)k * LABEL_SLOT, LABEL_SLOT);
             p += slen[k];
             if (v < 100) {
@@ -174,10 +180,18 @@ static void fmt_rows_slots(fmt_job *j)
         }
     }
     j->len = (size_t)(p - j->buf);
-    rows = j->rows + (j->beg - j->first) * (uint64_t)K;
-    for (n = j->beg; n < j->end; n++, rows += K)
-        for (k = 0; k < K; k++)
-            tot[k] += rows[k];
+    i = (j->beg - j->first) * (uint64_t)K;
+    for (n = j->beg; n < j->end; n++)
+        for (k = 0; k < K; k++, i++)
+            tot[k] += ROW_AT(rows, wide, i);
+}
+
+static void fmt_rows_slots(fmt_job *j)
+{
+    if (j->wide)
+        fmt_rows_slots_w(j, 1);
+    else
+        fmt_rows_slots_w(j, 0);
 }
 
 static void fmt_batch(fmt_job *j)
@@ -205,7 +219,7 @@ static void fmt_batch(fmt_job *j)
     }
     p = j->buf;
     for (n = j->beg; n < j->end; n++) {
-        const uint64_t *row = j->rows + (n - j->first) * (uint64_t)j->nrads;
+        const uint64_t r0 = (n - j->first) * (uint64_t)j->nrads;
         memcpy(p, "id ", 3);
         p = put_u64(p + 3, j->h->halo[n].id);
         *p++ = '\n';
@@ -215,9 +229,10 @@ static void fmt_batch(fmt_job *j)
                 __builtin_memcpy(p, l->s, LABEL_SLOT);
             else
                 memcpy(p, l->s, l->len);
-            p = put_u64(p + l->len, row[k]);
+            const uint64_t v = ROW_AT(j->rows, j->wide, r0 + (uint64_t)k);
+            p = put_u64(p + l->len, v);
             *p++ = '\n';
-            j->tot[k] += row[k];
+            j->tot[k] += v;
         }
     }
     j->len = (size_t)(p - j->buf);
@@ -235,21 +250,46 @@ static const uint64_t *rows_of_array(void *ctx, uint64_t beg, uint64_t n)
     return a->counts + beg * (uint64_t)a->nrads;
 }
 
+static int write_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows64,
+                      sjcs_rows32_fn rows32, void *ctx, const float *radiuses_sq, int nrads,
+                      int nthreads);
+
 int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *counts,
                       const float *radiuses_sq, int nrads, int nthreads)
 {
     array_rows a = {counts, nrads};
-    return sjcs_write_result_rows(fname, h, rows_of_array, &a, radiuses_sq, nrads, nthreads);
+    return write_rows(fname, h, rows_of_array, NULL, &a, radiuses_sq, nrads, nthreads);
 }
 
 int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows, void *ctx,
                            const float *radiuses_sq, int nrads, int nthreads)
 {
+    return write_rows(fname, h, rows, NULL, ctx, radiuses_sq, nrads, nthreads);
+}
+
+int sjcs_write_result_rows32(const char *fname, const sjcs_halos *h, sjcs_rows32_fn rows,
+                             void *ctx, const float *radiuses_sq, int nrads, int nthreads)
+{
+    return write_rows(fname, h, NULL, rows, ctx, radiuses_sq, nrads, nthreads);
+}
+
+/* The rows of halos [beg, beg + n) from whichever callback is set. */
+static const void *get_rows(sjcs_rows_fn rows64, sjcs_rows32_fn rows32, void *ctx, uint64_t beg,
+                            uint64_t n)
+{
+    return rows64 ? (const void *)rows64(ctx, beg, n) : (const void *)rows32(ctx, beg, n);
+}
+
+static int write_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows64,
+                      sjcs_rows32_fn rows32, void *ctx, const float *radiuses_sq, int nrads,
+                      int nthreads)
+{
     label_t *labels;
     char *slots;
     unsigned char *slot_len;
     uint64_t *total, *row0;
-    const uint64_t *cur, *next;
+    const void *cur, *next;
+    const int wide = rows64 != NULL;
     fmt_job *jobs;
     pthread_t *th;
     worker_arg *wargs;
@@ -351,16 +391,18 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
             break;
         nworkers++;
     }
-    cur = h->n ? rows(ctx, 0, h->n < BATCH_HALOS ? h->n : BATCH_HALOS) : NULL;
+    cur = h->n ? get_rows(rows64, rows32, ctx, 0, h->n < BATCH_HALOS ? h->n : BATCH_HALOS) : NULL;
     if (h->n && cur == NULL)
         rc = -1;
     else if (h->n)
-        memcpy(row0, cur, sizeof(uint64_t) * (size_t)nrads);
+        for (k = 0; k < nrads; k++)
+            row0[k] = ROW_AT(cur, wide, k);
     for (b = 0; b < h->n && rc == 0; b += BATCH_HALOS) {
         uint64_t e = b + BATCH_HALOS < h->n ? b + BATCH_HALOS : h->n;
         for (t = 0; t < nthreads; t++) {
             jobs[t].h = h;
             jobs[t].rows = cur;
+            jobs[t].wide = wide;
             jobs[t].first = b;
             jobs[t].labels = labels;
             jobs[t].slots = slots;
@@ -378,7 +420,7 @@ int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn 
         next = NULL;
         if (e < h->n) {
             uint64_t e2 = e + BATCH_HALOS < h->n ? e + BATCH_HALOS : h->n;
-            next = rows(ctx, e, e2 - e);
+            next = get_rows(rows64, rows32, ctx, e, e2 - e);
             if (next == NULL)
                 rc = -1;
         }

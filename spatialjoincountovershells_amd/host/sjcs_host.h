@@ -135,6 +135,10 @@ int sjcs_write_result(const char *fname, const sjcs_halos *h, const uint64_t *co
 typedef const uint64_t *(*sjcs_rows_fn)(void *ctx, uint64_t beg, uint64_t n);
 int sjcs_write_result_rows(const char *fname, const sjcs_halos *h, sjcs_rows_fn rows, void *ctx,
                            const float *radiuses_sq, int nrads, int nthreads);
+/* The same with u32 count rows (half the bytes to fetch and read). */
+typedef const uint32_t *(*sjcs_rows32_fn)(void *ctx, uint64_t beg, uint64_t n);
+int sjcs_write_result_rows32(const char *fname, const sjcs_halos *h, sjcs_rows32_fn rows,
+                             void *ctx, const float *radiuses_sq, int nrads, int nthreads);
 /* Halos per batch of the writer (the largest n it asks rows for). */
 #define SJCS_RESULT_BATCH 32768
 

@@ -485,7 +485,7 @@ static void rt_chunks(sjcs_engine *e, void *pin[2], const sjcs_cli *cli, char **
 
 typedef struct {
     sjcs_engine *e;
-    uint64_t *rows[2];  /* page-locked, one batch of rows each */
+    void *rows[2];  /* page-locked, one batch of rows each */
     int slot;
 } row_source;
 
@@ -494,6 +494,17 @@ static const uint64_t *fetch_rows(void *ctx, uint64_t beg, uint64_t n)
     row_source *r = ctx;
     r->slot ^= 1;
     if (sjcs_get_counts_range(r->e, beg, n, r->rows[r->slot]) != SJCS_OK) {
+        fprintf(stderr, "ERROR: counts: %s\n", sjcs_last_error(r->e));
+        return NULL;
+    }
+    return r->rows[r->slot];
+}
+
+static const uint32_t *fetch_rows32(void *ctx, uint64_t beg, uint64_t n)
+{
+    row_source *r = ctx;
+    r->slot ^= 1;
+    if (sjcs_get_counts_range_u32(r->e, beg, n, r->rows[r->slot]) != SJCS_OK) {
         fprintf(stderr, "ERROR: counts: %s\n", sjcs_last_error(r->e));
         return NULL;
     }
@@ -693,19 +704,25 @@ int main(int argc, char *argv[])
     print_ms(">>>> total searching time", &tvtotal);
 
     /* result (main.c:660-664): rows come back while earlier rows are
-     * formatted */
+     * formatted; as u32 (half the bytes over the link) while the totals are
+     * still the device's 32-bit running counts */
     TIMER_START(tv);
     src.e = e;
     src.slot = 1;
     src.rows[0] = src.rows[1] = NULL;
-    if (sjcs_host_alloc(sizeof(uint64_t) * SJCS_RESULT_BATCH * (size_t)cli.nrads,
-                        (void **)&src.rows[0]) != SJCS_OK ||
-        sjcs_host_alloc(sizeof(uint64_t) * SJCS_RESULT_BATCH * (size_t)cli.nrads,
-                        (void **)&src.rows[1]) != SJCS_OK)
-        die_engine(NULL, "counts staging");
-    if (sjcs_write_result_rows(cli.outfile, &halos, fetch_rows, &src, r2, cli.nrads, hthreads) <
-        0)
-        exit(EXIT_FAILURE);
+    {
+        const int narrow = sjcs_get_counts_range_u32(e, 0, 0, NULL) == SJCS_OK;
+        const size_t rb = (narrow ? sizeof(uint32_t) : sizeof(uint64_t)) * SJCS_RESULT_BATCH *
+                          (size_t)cli.nrads;
+        if (sjcs_host_alloc(rb, &src.rows[0]) != SJCS_OK ||
+            sjcs_host_alloc(rb, &src.rows[1]) != SJCS_OK)
+            die_engine(NULL, "counts staging");
+        if ((narrow ? sjcs_write_result_rows32(cli.outfile, &halos, fetch_rows32, &src, r2,
+                                               cli.nrads, hthreads)
+                    : sjcs_write_result_rows(cli.outfile, &halos, fetch_rows, &src, r2,
+                                             cli.nrads, hthreads)) < 0)
+            exit(EXIT_FAILURE);
+    }
     print_ms(">>>> result writing time", &tv);
     trace("result written");
 

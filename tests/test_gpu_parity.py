@@ -1162,6 +1162,49 @@ def test_counts_range_follows_the_totals(eng):
     assert int(eng.counts_range(100, 50).sum()) == 0
 
 
+def test_counts_range_u32(built, eng):
+    """The 32-bit rows equal the 64-bit ones while the totals are the
+    device's u32 running counts (one chunk, then a second one without a
+    fold), in any range and with the widths alternating; after a fold into
+    64 bits they are refused (SJCS_ERR_STATE) until a reset, after which they
+    are zeros and then the new chunk's counts."""
+    p, h = gen_uniform(37, 60_000, 5_000, face=0.2)
+    _, r2 = t.oracle_thresholds(0.01, 10.0, 12)
+    eng.set_shells(r2)
+    eng.set_halos(h)
+    assert eng.counts_fit_u32()
+    assert int(eng.counts_range_u32(0, 5000).sum()) == 0
+    eng.add_particles(p[:30_000])
+    eng.add_particles(p[30_000:])
+    assert eng.counts_fit_u32()
+    want, _ = oracle_chunks([p], h, r2)
+    got = eng.counts_range_u32(0, 5000)
+    assert got.dtype == np.uint32
+    assert_counts_equal(got.astype(np.uint64), want, h)
+    for a, b in ((0, 1), (1, 2500), (2500, 4999), (4999, 5000)):
+        assert np.array_equal(eng.counts_range(a, b - a), want[a:b])
+        assert np.array_equal(eng.counts_range_u32(a, b - a).astype(np.uint64), want[a:b])
+    assert eng.counts_range_u32(5000, 0).shape == (0, 12)
+    eng.set_option("fold_limit", 1)
+    try:
+        eng.reset_counts()
+        eng.add_particles(p[:30_000])
+        eng.add_particles(p[30_000:])  # folds the first chunk into 64 bits
+        assert not eng.counts_fit_u32()
+        with pytest.raises(built.SJCSError) as err:
+            eng.counts_range_u32(0, 10)
+        assert err.value.code == built.SJCS_ERR_STATE
+        assert_counts_equal(eng.counts_range(0, 5000), want, h)
+        eng.reset_counts()
+        assert eng.counts_fit_u32()
+        assert int(eng.counts_range_u32(0, 5000).sum()) == 0
+        eng.add_particles(p[:30_000])
+        one, _ = oracle_chunks([p[:30_000]], h, r2)
+        assert_counts_equal(eng.counts_range_u32(0, 5000).astype(np.uint64), one, h)
+    finally:
+        eng.set_option("fold_limit", 0)
+
+
 def test_full_size_config2_properties(eng):
     """Config 2 at its full size (1e8 particles x 1e7 halos x 40 shells),
     where the oracle is too slow: counts are independent of the cell size

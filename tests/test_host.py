@@ -217,6 +217,51 @@ def test_result_writer_many_batches(host, tmp_path, spec, nthreads, top):
     assert got == want
 
 
+ROWS32_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
+
+
+@pytest.mark.parametrize("spec,nthreads,top", [("0.001:5:40", 16, 150),
+                                               ("0.01:20:64", 3, 1 << 32),
+                                               ("100000:900000:3", 5, 1 << 32)])
+def test_result_writer_rows32(host, tmp_path, spec, nthreads, top):
+    """The writer fed u32 count rows through its callback (batch by batch,
+    the CLI's read-back of 32-bit totals) writes the same bytes as the u64
+    writer on the same counts: 140000 halos, small counts or counts up to
+    2^32 - 1, slot-table labels and longer ones."""
+    rng = np.random.default_rng(len(spec) + nthreads)
+    mn, mx, k = t.parse_radius(spec)
+    _, r2 = t.oracle_thresholds(mn, mx, k)
+    n = 140_000
+    ids = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    c32 = rng.integers(0, top, (n, k), dtype=np.uint64).astype(np.uint32)
+    c32[0, 0] = np.iinfo(np.uint32).max
+    c64 = c32.astype(np.uint64)
+    arr = (Halo * n)()
+    ctypes.memmove(arr, np.rec.fromarrays([ids, np.zeros(n, np.float32)] +
+                                          [np.zeros(n, np.float32)] * 3,
+                                          dtype=[("id", "<u8"), ("m", "<f4"), ("x", "<f4"),
+                                                 ("y", "<f4"), ("z", "<f4")]).tobytes(),
+                   ctypes.sizeof(Halo) * n)
+    hs = Halos(ctypes.cast(arr, ctypes.POINTER(Halo)), n)
+    asked = []
+
+    def rows(ctx, beg, cnt):
+        asked.append((beg, cnt))
+        return c32.ctypes.data + int(beg) * k * 4
+
+    cb = ROWS32_FN(rows)
+    host.sjcs_write_result_rows32.argtypes = [ctypes.c_char_p, ctypes.POINTER(Halos), ROWS32_FN,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                              ctypes.c_int]
+    out32, out64 = tmp_path / "r32.txt", tmp_path / "r64.txt"
+    assert host.sjcs_write_result_rows32(str(out32).encode(), ctypes.byref(hs), cb, None,
+                                         r2.ctypes.data, k, nthreads) == 0
+    assert host.sjcs_write_result(str(out64).encode(), ctypes.byref(hs), c64.ctypes.data,
+                                  r2.ctypes.data, k, nthreads) == 0
+    assert out32.read_bytes() == out64.read_bytes()
+    assert [a for a, _ in asked] == list(range(0, n, 32768))
+
+
 def test_particle_count_ignores_trailing_bytes(host, tmp_path):
     p = tmp_path / "p.dat"
     p.write_bytes(np.arange(30, dtype=np.float32).tobytes() + b"\x01\x02\x03\x04\x05")
