@@ -1224,6 +1224,10 @@ def test_full_size_config2_properties(eng):
         # the end, equal the same rows of the whole result
         for a, m in ((1_234_567, 300_001), (h.shape[0] - 250_000, 250_000)):
             assert np.array_equal(eng.counts_range(a, m), base[a:a + m])
+        # the 32-bit rows the CLI reads back at this size: the same values
+        assert eng.counts_fit_u32()
+        for a, m in ((0, 32_768), (1_234_567, 300_001), (h.shape[0] - 250_000, 250_000)):
+            assert np.array_equal(eng.counts_range_u32(a, m).astype(np.uint64), base[a:a + m])
         w = eng.last_chunk_pair_tests()
         assert abs(w / 1e9 - 1.0) < 0.005
         print("full size: W", w, "pairs", int(base.sum()), flush=True)
