@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref", "astr_rcount")
+RT_REF = os.path.join(ROOT, "oracle", "_ref", "rt_rcount")
 
 
 def write_halo_text(path, ids, pts):
@@ -30,9 +31,8 @@ def write_halo_text(path, ids, pts):
                      for i, (x, y, z) in zip(ids.tolist(), pts.tolist()))
 
 
-@pytest.mark.skipif(not os.path.exists(REF), reason="reference binary not built (oracle/ref.mk)")
-@pytest.mark.parametrize("seed", range(48))
-def test_cli_equals_reference_binary(built, seed, tmp_path):
+def draw_run(built, seed, tmp_path):
+    """Input files and argv of random case `seed` (without --output-file)."""
     rng = np.random.default_rng(90_000 + seed)
     nfiles = int(rng.integers(1, 4))
     compressed = rng.random() < 0.25
@@ -77,18 +77,41 @@ def test_cli_equals_reference_binary(built, seed, tmp_path):
         args += ["--backlog", str(backlog)]
     print(f"case {seed}: files {nfiles} compressed {compressed} halos {nh} radius {radius} "
           f"threads {thr} backlog {backlog}", flush=True)
+    return args
+
+
+def compare_runs(ref, ours, ours_path, args, tmp_path):
     ref_out, our_out = tmp_path / "ref.txt", tmp_path / "ours.txt"
-    r = subprocess.run([REF] + args + ["--output-file", str(ref_out)], capture_output=True,
+    r = subprocess.run([ref] + args + ["--output-file", str(ref_out)], capture_output=True,
                        text=True, timeout=120)
-    o = subprocess.run([built.CLI_PATH] + args + ["--output-file", str(our_out)],
+    o = subprocess.run([ours] + args + ["--output-file", str(our_out)],
                        capture_output=True, text=True, timeout=120)
     assert o.returncode == r.returncode, o.stderr[-500:]
     if r.returncode != 0:
         # an argv the reference refuses (e.g. --backlog with --thread not
         # above the index construction threads, main.c:576-589): the same
         # messages, the same usage text
-        assert o.stderr.replace(built.CLI_PATH, "ARGV0") == r.stderr.replace(REF, "ARGV0")
+        assert o.stderr.replace(ours_path, "ARGV0") == r.stderr.replace(ref, "ARGV0")
         assert o.stdout == r.stdout
         return
     a, b = ref_out.read_bytes(), our_out.read_bytes()
     assert len(a) > 0 and a == b, f"result files differ ({len(a)} vs {len(b)} bytes)"
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference binary not built (oracle/ref.mk)")
+@pytest.mark.parametrize("seed", range(48))
+def test_cli_equals_reference_binary(built, seed, tmp_path):
+    args = draw_run(built, seed, tmp_path)
+    compare_runs(REF, built.CLI_PATH, built.CLI_PATH, args, tmp_path)
+
+
+@pytest.mark.skipif(not os.path.exists(RT_REF), reason="reference binary not built (oracle/ref.mk)")
+@pytest.mark.parametrize("seed", range(24))
+def test_rt_cli_equals_reference_binary(built, seed, tmp_path):
+    """The CLI run as rt_rcount (a link of that name) against the
+    reference rt_rcount (src/rt_rcount/main.c) on the same random cases:
+    its R-tree counts, byte for byte."""
+    link = tmp_path / "rt_rcount"
+    os.symlink(built.CLI_PATH, link)
+    args = draw_run(built, 1000 + seed, tmp_path)
+    compare_runs(RT_REF, str(link), str(link), args, tmp_path)
